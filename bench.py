@@ -1,0 +1,227 @@
+"""Benchmark: the k=31 count map on one MI355X.
+
+Workload (the count-map configuration): a <31, u64, l=22> map sized for
+100M keys; a 100 Mbp seeded uniform genome; 20M reads of 150 bp (uniform
+starts, random strand, 0.5% substitutions, 0.05% N, lowercase runs in half
+the reads), upserted with +1 (absent keys start at 1) in batches of 65,536
+reads; then lookups of 2M fresh reads.  A step is one batch upsert.  Reads
+are generated on the device and resident in HBM before the timed region.
+
+Prints one JSON line: k-mer upserts/s over the timed steps, with lookups/s,
+the insert kernel's roofline position, and the reference CPU path timed on a
+bounded sample of the same reads on this host's cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+READ_LEN = 150
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes per k-mer of a u64 count-map upsert/lookup over 150-bp reads:
+# input 150/120 = 1.25 B, the bucket metadata line 64 B / 5.29 k-mers per
+# minimizer run = 12.10 B, and the 16-byte key + value slot.
+ALG_BYTES_PER_KMER = 150 / 120 + 64 / 5.29 + 16
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=None, help="timed batches (default: all after warmup)")
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--reads", type=int, default=20_000_000)
+    p.add_argument("--batch", type=int, default=65_536)
+    p.add_argument("--genome", type=int, default=100_000_000)
+    p.add_argument("--expected", type=int, default=100_000_000)
+    p.add_argument("--lookup-reads", type=int, default=2_000_000)
+    p.add_argument("--cpu-sample-reads", type=int, default=2_000_000)
+    p.add_argument("--cpu-threads", type=int, default=None)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--blocks-per-cu", type=int, default=0)
+    p.add_argument("--seed", type=int, default=0xC2)
+    return p.parse_args()
+
+
+def gen_text(api, capi, n_reads, genome_seed, genome_len, seed, sub=0.005, nrate=0.0005, lower=0.5):
+    """n_reads reads as a device text (each read followed by '\\n'), padded."""
+    L1 = READ_LEN + 1
+    buf = capi.DeviceBuffer(api, n_reads * L1 + capi.TEXT_PAD)
+    api.check(api.lib.kache_amd_synth_reads(buf.ptr, n_reads, READ_LEN, genome_seed, genome_len, seed, sub, nrate, lower), "synth_reads")
+    api.check(api.lib.kache_amd_device_fill(buf.ptr + n_reads * L1, ord("\n"), capi.TEXT_PAD), "fill")
+    return buf
+
+
+def valid_positions(host_rows):
+    """Number of 31-mer positions without a non-ACGT character, per read."""
+    bad = ~np.isin(host_rows | 0x20, np.frombuffer(b"acgt", np.uint8))
+    cs = np.concatenate([np.zeros((len(host_rows), 1), np.int32), np.cumsum(bad, axis=1, dtype=np.int32)], axis=1)
+    win = cs[:, 31:] - cs[:, :-31]
+    return (win == 0).sum(axis=1)
+
+
+def cpu_baseline(host_text, n_reads, expected, threads):
+    """The reference's C ABI (oracle/_ref, built from its sources) inserting
+    the same reads from `threads` threads, each its own user, in calls of
+    1,024 reads; timed over the inserts only."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from kache_hash_amd import capi
+    path = O.ref_library()
+    if path is None:
+        return None
+    api = capi.Api(path)
+    t = capi.Table(api, 8, expected, 0.8, False, threads)
+    users = [t.register() for _ in range(threads)]
+    L1 = READ_LEN + 1
+    per_call = 1024
+    calls = [(a, min(per_call, n_reads - a)) for a in range(0, n_reads, per_call)]
+    base = host_text.ctypes.data
+    lens = np.full(per_call, READ_LEN, dtype=np.uint64)
+    ptr_tabs = [(np.arange(c, dtype=np.uint64) * L1 + np.uint64(base + a * L1)).astype(np.uint64) for a, c in calls]
+    totals = [0] * threads
+
+    def work(i):
+        n = C.c_uint64()
+        for j in range(i, len(calls), threads):
+            a, c = calls[j]
+            st = api.lib.kache_insert(t.h, users[i], ptr_tabs[j].ctypes.data, lens.ctypes.data, c, capi.ADD, 1, C.byref(n))
+            assert st == 0
+            totals[i] += n.value
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.perf_counter() - t0
+    rows = host_text[: n_reads * L1].reshape(n_reads, L1)[:, :READ_LEN]
+    kmers = int(valid_positions(rows).sum())
+    size = t.size()
+    t.close()
+    return {"value": kmers / dt, "unit": "k-mer upserts/s", "cores": threads, "kind": "reference",
+            "sample": f"first {n_reads} reads ({kmers} k-mers, {size} distinct) upserted into a fresh 100M-key u64 map "
+                      f"by {threads} threads in calls of {per_call} reads; {os.path.basename(path)}",
+            "seconds": dt}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+              "the sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import kache_hash_amd as KA
+    from kache_hash_amd import capi
+
+    api = KA.api()
+    nb = (a.reads + a.batch - 1) // a.batch
+    steps = a.steps if a.steps is not None else max(nb - a.warmup, 1)
+    need = a.warmup + steps
+    n_reads = min(a.reads, need * a.batch)
+    t_gen = time.perf_counter()
+    # Reads come straight from the seeded genome's hash (the genome itself is
+    # never materialized: kache_amd_synth_reads derives each window from it).
+    text = gen_text(api, capi, n_reads, a.seed, a.genome, a.seed + 1)
+    look = gen_text(api, capi, a.lookup_reads, a.seed, a.genome, a.seed + 2)
+    print(f"bench: generated {n_reads} reads + {a.lookup_reads} lookup reads in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    t = KA.table(width=8, expected_keys=a.expected)
+    if a.blocks_per_cu:
+        t.set_blocks_per_cu(a.blocks_per_cu)
+    u = t.register()
+    L1 = READ_LEN + 1
+    base = text.ptr
+
+    def step(i):
+        r0 = (i * a.batch) % n_reads
+        c = min(a.batch, n_reads - r0)
+        return t.insert_device(base + r0 * L1, c * L1, KA.ADD, 1, u)
+
+    for i in range(a.warmup):
+        step(i)
+    t.profile(True)
+    t0 = time.perf_counter()
+    for i in range(a.warmup, a.warmup + steps):
+        step(i)
+    dt = time.perf_counter() - t0    # each step returns after its device work
+    ins_ms, ins_launches = t.kernel_time("insert")
+    split_ms, _ = t.kernel_time("split")
+    re_ms, _ = t.kernel_time("reinsert")
+    t.profile(False)
+
+    # k-mer positions of the timed steps: every one is present after the inserts.
+    r0, r1 = a.warmup * a.batch, min(n_reads, (a.warmup + steps) * a.batch)
+    timed_kmers = t.count_device(base + r0 * L1, (r1 - r0) * L1, positions=True)[1] if r1 > r0 else 0
+
+    # Lookups: 2M fresh reads.
+    t.profile(True)
+    t1 = time.perf_counter()
+    hits = look_kmers = 0
+    for r in range(0, a.lookup_reads, a.batch):
+        c = min(a.batch, a.lookup_reads - r)
+        h, p = t.count_device(look.ptr + r * L1, c * L1, positions=True)
+        hits += h
+        look_kmers += p
+    dt_look = time.perf_counter() - t1
+    q_ms, q_launches = t.kernel_time("query")
+    t.profile(False)
+    info = t.info()
+    stats = t.stats()
+
+    per_launch_kmers = timed_kmers / max(ins_launches, 1)
+    avg_ms = ins_ms / max(ins_launches, 1)
+    achieved = per_launch_kmers * ALG_BYTES_PER_KMER / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "insert_count_map_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        m = min(a.cpu_sample_reads, n_reads)
+        host = text.download(m * L1)
+        print(f"bench: reference CPU path on {m} reads, {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(host, m, a.expected, threads)
+
+    value = timed_kmers / dt
+    out = {
+        "metric": "k-mer upserts/s + lookups/s (k=31 count map); p50/p99 served-batch latency",
+        "value": value, "unit": "k-mer upserts/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic (seeded on device)",
+        "config": {"workload": "count map <31,u64,22>: 100 Mbp genome, 150-bp reads (0.5% subst, 0.05% N, mixed case), "
+                               "upsert +1 in batches of 65,536 reads, lookups of 2M fresh reads",
+                   "reads": n_reads, "batch_reads": a.batch, "expected_keys": a.expected, "genome_bp": a.genome},
+        "lookups_per_s": look_kmers / dt_look,
+        "lookup_hits": hits, "lookup_kmers": look_kmers,
+        "timed_kmers": timed_kmers,
+        "keys": stats["keys"], "capacity": stats["capacity"], "resizes": info["resizes"],
+        "placement_pct": [stats["primary_pct"], stats["secondary_pct"], stats["overflow_pct"]],
+        "kernel_ms": {"insert": ins_ms, "split": split_ms, "reinsert": re_ms, "query": q_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "k_insert<map, ADD>", "alg_bytes_per_kmer": ALG_BYTES_PER_KMER,
+                     "kmers_per_launch": per_launch_kmers, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

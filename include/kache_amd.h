@@ -1,0 +1,103 @@
+#ifndef KACHE_AMD_H
+#define KACHE_AMD_H
+
+/*
+ * MI355X extensions of the kache-hash C ABI (include/kache_capi.h).  None of
+ * these has a counterpart in the reference: they serve callers whose
+ * sequences already sit in device memory (batch pipelines, the serving loop,
+ * benchmarks), measurement, and tests that check device code on the host.
+ *
+ * Device texts.  A device text is `text_bytes` bytes of concatenated
+ * sequences in device memory, each sequence followed by at least one
+ * non-ACGT byte, so that no k-mer spans two of them.  The allocation must be
+ * 16-byte aligned and readable for KACHE_AMD_TEXT_PAD bytes past text_bytes;
+ * those bytes never count as bases.  The calls order themselves after all
+ * prior work of the device (they synchronize the device before starting), and
+ * return after their own work is done.
+ */
+
+#include "kache_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KACHE_AMD_TEXT_PAD 512
+
+/* kache_insert / kache_insert_oriented / kache_update over a device text. */
+int kache_amd_insert_device(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x_fwd, uint64_t x_rev, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
+
+/* The number of k-mer positions of a device text present in the table
+ * (`hits`) and, unless NULL, of all its k-mer positions (`positions`). */
+int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text_bytes, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;
+
+/* The C++ API's wrapping `v + x` (a map's `upsert(..., [](v){ return v + x; }, x)`,
+ * include/Streaming_Kmer_Hash_Table.hpp:672), which the C ABI's saturating
+ * KACHE_ADD does not cover: op code for the calls above and kache_insert. */
+#define KACHE_AMD_ADD_WRAP 6
+
+/* Device describer, for tests: for each position p of [0, text_bytes) puts
+ * whether a 31-mer starts there (valid), its canonical key, minimizer hash,
+ * checksum and orientation.  All pointers are device pointers. */
+int kache_amd_describe_device(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) KACHE_NOEXCEPT;
+
+/* Kernel timing: with profiling on, HIP events bracket every launch on the
+ * table's stream; kache_amd_kernel_time puts the summed milliseconds and the
+ * number of launches of the kernel family `name` ("insert", "update",
+ * "query", "split", "reinsert", "ov_rehash", "collect", "reduce"). */
+int kache_amd_profile(kache_table* t, int on) KACHE_NOEXCEPT;
+int kache_amd_kernel_time(const kache_table* t, const char* name, double* ms, uint64_t* launches) KACHE_NOEXCEPT;
+
+/* Blocks per compute unit of the persistent launches (0: as many as fit). */
+int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
+
+/* Table geometry and history. */
+typedef struct kache_amd_info
+{
+    uint64_t buckets;
+    uint64_t bucket_bytes;
+    uint64_t overflow_slots;
+    uint64_t overflow_keys;
+    uint64_t resizes;
+    uint64_t device_bytes;      /* bucket store + overflow table */
+} kache_amd_info;
+int kache_amd_get_info(const kache_table* t, kache_amd_info* out) KACHE_NOEXCEPT;
+
+/* Device memory for device texts and outputs (hipMalloc'd; synchronous copies). */
+int kache_amd_device_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
+int kache_amd_device_free(void* ptr) KACHE_NOEXCEPT;
+int kache_amd_copy(void* dst, const void* src, uint64_t bytes, int to_device) KACHE_NOEXCEPT;
+int kache_amd_device_fill(void* ptr, uint8_t value, uint64_t bytes) KACHE_NOEXCEPT;
+
+/* Seeded synthetic workloads, made on the device from a counter-based hash:
+ * a genome of n uniform ACGT bases; and n_reads reads of read_len bases from
+ * it as a device text (each read followed by '\n'): uniform starts, random
+ * strand, substitutions at rate `sub`, N at rate `n_rate`, and a lowercase
+ * run in a fraction `lower` of the reads.  kache_amd_host_synth_reads writes
+ * reads [first_read, first_read + n_reads) of the same set on the host. */
+int kache_amd_synth_genome(char* out, uint64_t n, uint64_t seed) KACHE_NOEXCEPT;
+int kache_amd_synth_reads(char* out, uint64_t n_reads, uint64_t read_len, uint64_t genome_seed, uint64_t genome_len,
+                          uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
+int kache_amd_host_synth_reads(char* out, uint64_t first_read, uint64_t n_reads, uint64_t read_len, uint64_t genome_seed,
+                               uint64_t genome_len, uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
+const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
+
+/* Description of the last failure of a call on this thread. */
+const char* kache_amd_last_error(void) KACHE_NOEXCEPT;
+
+/* Host runs of the device's own hash and sizing code, for tests. */
+uint64_t kache_amd_host_minimizer_hash(uint64_t key) KACHE_NOEXCEPT;
+uint8_t kache_amd_host_checksum(uint64_t key) KACHE_NOEXCEPT;
+uint64_t kache_amd_host_xxh3(uint64_t x, uint64_t seed) KACHE_NOEXCEPT;
+uint64_t kache_amd_host_cap_for(uint64_t n, double lf, uint32_t pow2) KACHE_NOEXCEPT;
+/* Describes a host text the way one device tile does (packing, windows and
+ * window minima), position by position; same outputs as the device call. */
+int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) KACHE_NOEXCEPT;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

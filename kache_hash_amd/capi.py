@@ -1,0 +1,370 @@
+"""ctypes bindings of the kache-hash C ABI (include/kache_capi.h).
+
+The same bindings drive any library exporting that ABI: this package's
+libkache_amd.so (the MI355X table) or a build of the reference's
+capi/kache_capi.cpp.  `Table` mirrors the reference's C interface one call
+per method, with its argument meanings, status codes and threading contract
+(capi/kache_capi.h:111-202); the Rust crate's Builder/Inserter/Table phases
+(rust/kache-hash-sys/src/lib.rs:264-587) map onto it the same way.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+KACHE_OK, KACHE_EINVAL, KACHE_EUSERS, KACHE_ENOMEM, KACHE_EINTERNAL = range(5)
+KACHE_MAX_USERS = 128
+KEEP, ASSIGN, ADD, OR, MIN, MAX = range(6)
+ADD_WRAP = 6     # KACHE_AMD_ADD_WRAP: the C++ API's wrapping v + x (device library only)
+REDUCE_KEYS, REDUCE_POPCOUNT, REDUCE_SUM = range(3)
+ABSENT, PRESENT, NOT_KMER = range(3)
+K = 31
+TEXT_PAD = 512   # KACHE_AMD_TEXT_PAD
+
+
+class Options(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("value_bytes", C.c_uint32), ("expected_keys", C.c_uint64),
+                ("load_factor", C.c_double), ("resize_workers", C.c_uint32), ("pow2_capacity", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("capacity", C.c_uint64), ("bytes", C.c_uint64), ("keys", C.c_uint64), ("overflow_keys", C.c_uint64),
+                ("primary_pct", C.c_double), ("secondary_pct", C.c_double), ("overflow_pct", C.c_double)]
+
+
+class Info(C.Structure):
+    _fields_ = [("buckets", C.c_uint64), ("bucket_bytes", C.c_uint64), ("overflow_slots", C.c_uint64),
+                ("overflow_keys", C.c_uint64), ("resizes", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+KEYS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_size_t)
+
+_P = C.c_void_p
+_SEQS = C.c_void_p   # const char* const*
+_LENS = C.c_void_p   # const size_t*
+
+CAPI_SYMBOLS = {
+    "kache_supports": (C.c_int, [C.c_uint32, C.c_uint32]),
+    "kache_default_options": (Options, [C.c_uint32, C.c_uint32]),
+    "kache_new": (C.c_int, [C.POINTER(Options), C.POINTER(_P)]),
+    "kache_free": (None, [_P]),
+    "kache_register": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "kache_insert": (C.c_int, [_P, C.c_uint32, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_insert_oriented": (C.c_int, [_P, C.c_uint32, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_update": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_count": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P]),
+    "kache_count_masked": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, C.c_uint64, C.c_uint64, _P]),
+    "kache_sum": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P]),
+    "kache_lookup": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P, _P, _P]),
+    "kache_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64)]),
+    "kache_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "kache_reduce": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "kache_for_each": (C.c_int, [_P, C.c_uint32, C.c_uint32, KEYS_FN, _P]),
+    "kache_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "kache_strerror": (C.c_char_p, [C.c_int]),
+}
+
+AMD_SYMBOLS = {
+    "kache_amd_insert_device": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_insert_oriented_device": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_update_device": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_count_device": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "kache_amd_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    "kache_amd_device_free": (C.c_int, [_P]),
+    "kache_amd_copy": (C.c_int, [_P, _P, C.c_uint64, C.c_int]),
+    "kache_amd_device_fill": (C.c_int, [_P, C.c_uint8, C.c_uint64]),
+    "kache_amd_synth_genome": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
+    "kache_amd_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
+    "kache_amd_host_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
+    "kache_amd_synth_error": (C.c_char_p, []),
+    "kache_amd_describe_device": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P]),
+    "kache_amd_profile": (C.c_int, [_P, C.c_int]),
+    "kache_amd_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kache_amd_set_blocks_per_cu": (C.c_int, [_P, C.c_uint32]),
+    "kache_amd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
+    "kache_amd_last_error": (C.c_char_p, []),
+    "kache_amd_host_minimizer_hash": (C.c_uint64, [C.c_uint64]),
+    "kache_amd_host_checksum": (C.c_uint8, [C.c_uint64]),
+    "kache_amd_host_xxh3": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "kache_amd_host_cap_for": (C.c_uint64, [C.c_uint64, C.c_double, C.c_uint32]),
+    "kache_amd_host_describe": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P]),
+}
+
+
+class KacheError(RuntimeError):
+    def __init__(self, status, what, detail=""):
+        super().__init__(f"{what}: status {status}" + (f" ({detail})" if detail else ""))
+        self.status = status
+
+
+class Api:
+    """A loaded library exporting the kache C ABI (and, for ours, the extensions)."""
+
+    def __init__(self, path):
+        self.path = path
+        self.lib = C.CDLL(path)
+        for name, (res, args) in CAPI_SYMBOLS.items():
+            f = getattr(self.lib, name)
+            f.restype, f.argtypes = res, args
+        self.amd = hasattr(self.lib, "kache_amd_insert_device")
+        if self.amd:
+            for name, (res, args) in AMD_SYMBOLS.items():
+                f = getattr(self.lib, name)
+                f.restype, f.argtypes = res, args
+
+    def last_error(self):
+        return self.lib.kache_amd_last_error().decode() if self.amd else ""
+
+    def check(self, status, what):
+        if status != KACHE_OK:
+            raise KacheError(status, what, self.last_error())
+
+
+class Batch:
+    """Sequences as the C ABI takes them: an array of pointers and lengths,
+    here into one contiguous buffer (kept alive by the batch)."""
+
+    def __init__(self, seqs):
+        seqs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+        self.n = len(seqs)
+        self.lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        starts = np.zeros(self.n + 1, dtype=np.uint64)
+        if self.n:
+            np.cumsum(self.lens, out=starts[1:])
+        self.buf = np.frombuffer(b"".join(seqs) + b"\0", dtype=np.uint8).copy()
+        self.ptrs = (starts[:-1] + np.uint64(self.buf.ctypes.data)).astype(np.uint64)
+        self.seqs = seqs
+
+    @classmethod
+    def from_buffer(cls, buf, starts, lens):
+        b = cls.__new__(cls)
+        b.buf = buf
+        b.n = len(lens)
+        b.lens = np.ascontiguousarray(lens, dtype=np.uint64)
+        b.ptrs = (np.asarray(starts, dtype=np.uint64) + np.uint64(buf.ctypes.data)).astype(np.uint64)
+        b.seqs = None
+        return b
+
+    def positions(self):
+        l = self.lens.astype(np.int64)
+        return int(np.maximum(l - (K - 1), 0).sum())
+
+    @property
+    def c_seqs(self):
+        return self.ptrs.ctypes.data if self.n else None
+
+    @property
+    def c_lens(self):
+        return self.lens.ctypes.data if self.n else None
+
+
+def as_batch(seqs):
+    return seqs if isinstance(seqs, Batch) else Batch(seqs)
+
+
+class Table:
+    """A table behind the C ABI: a set (width 0) or a map to values of
+    `width` bytes, sized for `expected_keys` at `load_factor`."""
+
+    def __init__(self, api, width=0, expected_keys=1 << 22, load_factor=0.8, pow2=False, workers=16):
+        self.api = api
+        self.width = width
+        o = api.lib.kache_default_options(K, width)
+        o.expected_keys, o.load_factor, o.resize_workers, o.pow2_capacity = expected_keys, load_factor, workers, int(pow2)
+        h = C.c_void_p()
+        api.check(api.lib.kache_new(C.byref(o), C.byref(h)), "kache_new")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.api.lib.kache_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def register(self):
+        u = C.c_uint32()
+        self.api.check(self.api.lib.kache_register(self.h, C.byref(u)), "kache_register")
+        return u.value
+
+    def insert(self, seqs, op=KEEP, x=0, user=0):
+        b = as_batch(seqs)
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_insert(self.h, user, b.c_seqs, b.c_lens, b.n, op, x, C.byref(n)), "kache_insert")
+        return n.value
+
+    def insert_oriented(self, seqs, op, x_fwd, x_rev, user=0):
+        b = as_batch(seqs)
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_insert_oriented(self.h, user, b.c_seqs, b.c_lens, b.n, op, x_fwd, x_rev, C.byref(n)), "kache_insert_oriented")
+        return n.value
+
+    def update(self, seqs, op, x):
+        b = as_batch(seqs)
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_update(self.h, b.c_seqs, b.c_lens, b.n, op, x, C.byref(n)), "kache_update")
+        return n.value
+
+    def _per_seq(self, fn, b, *extra):
+        out = np.zeros(max(b.n, 1), dtype=np.uint64)
+        self.api.check(fn(self.h, b.c_seqs, b.c_lens, b.n, *extra, out.ctypes.data), fn.__name__)
+        return out[:b.n]
+
+    def count(self, seqs):
+        return self._per_seq(self.api.lib.kache_count, as_batch(seqs))
+
+    def count_masked(self, seqs, mask_fwd, mask_rev):
+        return self._per_seq(self.api.lib.kache_count_masked, as_batch(seqs), mask_fwd, mask_rev)
+
+    def sum(self, seqs):
+        return self._per_seq(self.api.lib.kache_sum, as_batch(seqs))
+
+    def lookup(self, seqs, values=True, fwd=True):
+        b = as_batch(seqs)
+        p = b.positions()
+        found = np.zeros(max(p, 1), dtype=np.uint8)
+        vals = np.zeros(max(p, 1) * max(self.width, 1), dtype=np.uint8) if (values and self.width) else None
+        fw = np.zeros(max(p, 1), dtype=np.uint8) if fwd else None
+        self.api.check(self.api.lib.kache_lookup(self.h, b.c_seqs, b.c_lens, b.n, found.ctypes.data,
+                                                 vals.ctypes.data if vals is not None else None,
+                                                 fw.ctypes.data if fw is not None else None), "kache_lookup")
+        v = None
+        if vals is not None:
+            v = vals[:p * self.width].view({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[self.width]).astype(np.uint64)
+        return found[:p], v, (fw[:p] if fw is not None else None)
+
+    def get(self, kmer):
+        f = C.c_uint8()
+        v = C.c_uint64()
+        kmer = kmer.encode() if isinstance(kmer, str) else bytes(kmer)
+        self.api.check(self.api.lib.kache_get(self.h, kmer, C.byref(f), C.byref(v)), "kache_get")
+        return bool(f.value), v.value
+
+    def size(self):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_size(self.h, C.byref(n)), "kache_size")
+        return n.value
+
+    def reduce(self, what, mask=~0 & (2**64 - 1), workers=1):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_reduce(self.h, what, mask, workers, C.byref(n)), "kache_reduce")
+        return n.value
+
+    def for_each(self, shards=1, shard=0, stop_after=None):
+        """Returns (keys, values, chunks) of the share, sorted by key."""
+        keys, vals = [], []
+        chunks = [0]
+        w = self.width
+
+        def cb(ctx, kp, vp, n):
+            keys.append(np.ctypeslib.as_array(kp, shape=(n,)).copy())
+            if vp and w:
+                raw = np.ctypeslib.as_array(C.cast(vp, C.POINTER(C.c_uint8)), shape=(n * w,)).copy()
+                vals.append(raw.view({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[w]).astype(np.uint64))
+            chunks[0] += 1
+            return 1 if (stop_after is not None and chunks[0] >= stop_after) else 0
+
+        fn = KEYS_FN(cb)
+        self.api.check(self.api.lib.kache_for_each(self.h, shards, shard, fn, None), "kache_for_each")
+        k = np.concatenate(keys) if keys else np.zeros(0, np.uint64)
+        v = np.concatenate(vals) if vals else np.zeros(len(k), np.uint64)
+        o = np.argsort(k, kind="stable")
+        return k[o], v[o], chunks[0]
+
+    def stats(self):
+        s = Stats()
+        self.api.check(self.api.lib.kache_get_stats(self.h, C.byref(s)), "kache_get_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    # --- device extensions (this package's library only) ---
+
+    def info(self):
+        i = Info()
+        self.api.check(self.api.lib.kache_amd_get_info(self.h, C.byref(i)), "kache_amd_get_info")
+        return {f: getattr(i, f) for f, _ in Info._fields_}
+
+    def insert_device(self, text_ptr, text_bytes, op=KEEP, x=0, user=0):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_insert_device(self.h, user, text_ptr, text_bytes, op, x, C.byref(n)), "kache_amd_insert_device")
+        return n.value
+
+    def insert_oriented_device(self, text_ptr, text_bytes, op, x_fwd, x_rev, user=0):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_insert_oriented_device(self.h, user, text_ptr, text_bytes, op, x_fwd, x_rev, C.byref(n)), "kache_amd_insert_oriented_device")
+        return n.value
+
+    def update_device(self, text_ptr, text_bytes, op, x):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_update_device(self.h, text_ptr, text_bytes, op, x, C.byref(n)), "kache_amd_update_device")
+        return n.value
+
+    def count_device(self, text_ptr, text_bytes, positions=False):
+        n, p = C.c_uint64(), C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_count_device(self.h, text_ptr, text_bytes, C.byref(n), C.byref(p)), "kache_amd_count_device")
+        return (n.value, p.value) if positions else n.value
+
+    def profile(self, on=True):
+        self.api.check(self.api.lib.kache_amd_profile(self.h, int(on)), "kache_amd_profile")
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)), "kache_amd_kernel_time")
+        return ms.value, n.value
+
+    def set_blocks_per_cu(self, blocks):
+        self.api.check(self.api.lib.kache_amd_set_blocks_per_cu(self.h, blocks), "kache_amd_set_blocks_per_cu")
+
+
+class DeviceBuffer:
+    """Device memory from the library (hipMalloc), freed with the object."""
+
+    def __init__(self, api, nbytes):
+        self.api, self.nbytes = api, nbytes
+        p = C.c_void_p()
+        api.check(api.lib.kache_amd_device_alloc(nbytes, C.byref(p)), "kache_amd_device_alloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            self.api.lib.kache_amd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, data, offset=0):
+        b = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else data)
+        self.api.check(self.api.lib.kache_amd_copy(self.ptr + offset, b.ctypes.data, b.nbytes, 1), "kache_amd_copy")
+
+    def download(self, nbytes=None, offset=0, dtype=np.uint8):
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n, np.uint8)
+        self.api.check(self.api.lib.kache_amd_copy(out.ctypes.data, self.ptr + offset, n, 0), "kache_amd_copy")
+        return out.view(dtype)
+
+
+def device_text(api, seqs):
+    """Sequences as a device text: each followed by a separator, padded.
+    Returns (buffer, text_bytes)."""
+    t = b"".join(bytes(s) + b"\n" for s in seqs)
+    buf = DeviceBuffer(api, len(t) + TEXT_PAD)
+    buf.upload(t + b"\n" * TEXT_PAD)
+    return buf, len(t)
+
+
+def library_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkache_amd.so")

@@ -1,0 +1,561 @@
+// Device side of the MI355X k-mer table: the fused describer (2-bit packing,
+// rolling canonical 31-mers, 22-mer Wang hashes, window-10 minimum), the
+// bucket probes, the value atomics and the kernels built from them.
+//
+// Layout in HBM (one table):
+//   bucket b, at buckets + b * bucket_bytes:
+//     [ 0, 32)  checksum of slot j (0 = empty)          include/Streaming_Kmer_Hash_Table.hpp:91-97
+//     [32, 64)  flag bits 1..255: the spill filter       (:1063-1093)
+//     [64, ..)  32 slots: stored key (~key, 0 = empty) [+ value u64]
+//   overflow: power-of-two array of slots {~key [, value]}, linear probing from
+//   XXH3_64bits(key) (include/Concurrent_Hash_Table.hpp:28, 43-52, 205-300).
+// Values of every width sit zero-extended in a u64, so one 64-bit atomic
+// updates any of them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kamd
+{
+
+constexpr int K = 31;
+constexpr int L = 22;
+constexpr int W = K - L + 1;                        // l-mers per k-mer
+constexpr uint32_t B = 32;                          // slots per bucket
+constexpr uint64_t KMER_MASK = (uint64_t(1) << (2 * K)) - 1;
+constexpr uint64_t LMER_MASK = (uint64_t(1) << (2 * L)) - 1;
+constexpr int TILE = 256;                           // positions per tile = threads per block
+constexpr int TILE_WORDS = 10;                      // 32-base words staged per tile: TILE + K + W - 1 bases
+constexpr uint32_t META_BYTES = 64;
+constexpr uint64_t TEXT_PAD = 512;                  // readable non-ACGT bytes required past a text's end
+
+enum Op : uint32_t { OP_KEEP = 0, OP_ASSIGN = 1, OP_ADD = 2, OP_OR = 3, OP_MIN = 4, OP_MAX = 5, OP_ADD_WRAP = 6 };
+
+// ---------------------------------------------------------------------------
+// Encoding and hashes (host and device, so host tests run the same code).
+
+// include/Kmer_Batch.hpp:24-35.
+__host__ __device__ __forceinline__ uint64_t wang_u64(uint64_t x)
+{
+    x = (~x) + (x << 21);
+    x = x ^ (x >> 24);
+    x = (x + (x << 3)) + (x << 8);
+    x = x ^ (x >> 14);
+    x = (x + (x << 2)) + (x << 4);
+    x = x ^ (x >> 28);
+    x = x + (x << 31);
+    return x;
+}
+
+// include/Kmer_Batch.hpp:44-67, the Wang variant (AVX-512 builds): the top
+// byte of wang_u64(key), never 0.
+__host__ __device__ __forceinline__ uint8_t checksum(uint64_t key)
+{
+    const uint8_t c = static_cast<uint8_t>(wang_u64(key) >> 56);
+    return c ? c : 1;
+}
+
+// Reverses the order of the 32 two-bit groups of x.
+__host__ __device__ __forceinline__ uint64_t rev_pairs(uint64_t x)
+{
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    return __builtin_bswap64(x);
+}
+
+// Minimizer hash of the canonical 31-mer `key` from its own l-mers
+// (include/Kmer_Lanes.hpp:247-274): min over offsets p of
+// wang(min(fwd l-mer at p, reverse complement of it)).
+__host__ __device__ __forceinline__ uint64_t minimizer_hash(uint64_t key)
+{
+    const uint64_t rc = rev_pairs(~(key << 2)) & KMER_MASK;
+    uint64_t m = ~uint64_t(0);
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+    {
+        const uint64_t lf = (key >> (2 * (K - L - p))) & LMER_MASK;
+        const uint64_t lr = (rc >> (2 * p)) & LMER_MASK;
+        const uint64_t hv = wang_u64(lf < lr ? lf : lr);
+        m = hv < m ? hv : m;
+    }
+    return m;
+}
+
+// XXH3_64bits_withSeed(&x, 8, seed) of xxHash 0.8.2: the 4-8 byte path
+// (include/xxHash/xxhash.h:4273-4285) and XXH3_rrmxmx (:4206-4213), with the
+// secret's bytes 8..23 (:3975) folded into one constant.
+__host__ __device__ __forceinline__ uint64_t xxh3_8(uint64_t x, uint64_t seed)
+{
+    constexpr uint64_t secret_8 = 0x1cad21f72c81017cull, secret_16 = 0xdb979083e96dd4deull;
+    const uint32_t lo = static_cast<uint32_t>(seed);
+    const uint32_t swapped = (lo >> 24) | ((lo >> 8) & 0xFF00u) | ((lo << 8) & 0xFF0000u) | (lo << 24);
+    seed ^= static_cast<uint64_t>(swapped) << 32;
+    const uint64_t input64 = (x >> 32) + (static_cast<uint64_t>(static_cast<uint32_t>(x)) << 32);
+    uint64_t h = input64 ^ ((secret_8 ^ secret_16) - seed);
+    constexpr uint64_t prime_mx2 = 0x9FB21C651E98DF25ull;
+    h ^= ((h << 49) | (h >> 15)) ^ ((h << 24) | (h >> 40));
+    h *= prime_mx2;
+    h ^= (h >> 35) + 8;
+    h *= prime_mx2;
+    return h ^ (h >> 28);
+}
+
+// include/Streaming_Kmer_Hash_Table.hpp:1014-1019.
+__host__ __device__ __forceinline__ uint64_t bucket_of(uint64_t h, uint64_t cap) { return ((h & 0xFFFFFFFFull) * cap) >> 32; }
+
+// include/Streaming_Kmer_Hash_Table.hpp:1055-1060: high word of x * cap.
+__host__ __device__ __forceinline__ uint64_t pick_bucket(uint64_t x, uint64_t cap)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(x, cap);
+#else
+    return static_cast<uint64_t>((static_cast<unsigned __int128>(x) * cap) >> 64);
+#endif
+}
+
+// include/Streaming_Kmer_Hash_Table.hpp:1063-1068.
+__host__ __device__ __forceinline__ uint32_t filter_pos(uint64_t h)
+{
+    const uint32_t x = static_cast<uint32_t>((h >> 32) & 0xFF);
+    return x + (x == 0);
+}
+
+// include/Streaming_Kmer_Hash_Table.hpp:2111-2115.
+__host__ __device__ __forceinline__ void secondaries(uint64_t h, uint64_t cap, uint64_t& p, uint64_t& q)
+{
+    p = pick_bucket(xxh3_8(h, 0), cap);
+    q = pick_bucket(xxh3_8(h, uint64_t(1) << 63), cap);
+}
+
+// Descriptor pieces of the 32-base window `w` (first base most significant)
+// starting at a position: its 31-mer's forward and reverse-complement forms
+// and the hash of its first l-mer's canonical form.
+__host__ __device__ __forceinline__ void window_parts(uint64_t w, uint64_t& kf, uint64_t& kr, uint64_t& lh)
+{
+    const uint64_t r = rev_pairs(~w);
+    kf = w >> 2;
+    kr = r & KMER_MASK;
+    const uint64_t lf = w >> (2 * (32 - L)), lr = r & LMER_MASK;
+    lh = wang_u64(lf < lr ? lf : lr);
+}
+
+// 2-bit code of an ACGT character in either case (include/Kmer_Batch.hpp:108).
+__host__ __device__ __forceinline__ uint32_t code_of(uint32_t ch) { return ((ch >> 2) ^ (ch >> 1)) & 3u; }
+
+// Whether a character is ACGT in either case (include/DNA_Utility.hpp:199-224).
+__host__ __device__ __forceinline__ uint32_t is_base(uint32_t ch)
+{
+    const uint32_t x = ch | 0x20u;
+    return (x == 'a') | (x == 'c') | (x == 'g') | (x == 't');
+}
+
+// Packs 32 characters (8 little-endian words) into their 2-bit codes, first
+// character most significant, and a mask of the non-ACGT ones (bit j: char j).
+__host__ __device__ __forceinline__ void pack32(const uint32_t (&d)[8], uint64_t& pk, uint32_t& bad)
+{
+    pk = 0, bad = 0;
+#pragma unroll
+    for(int i = 0; i < 8; ++i)
+#pragma unroll
+        for(int b = 0; b < 4; ++b)
+        {
+            const uint32_t ch = (d[i] >> (8 * b)) & 0xFFu;
+            const int j = 4 * i + b;
+            pk |= static_cast<uint64_t>(code_of(ch)) << (2 * (31 - j));
+            bad |= (is_base(ch) ^ 1u) << j;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Device memory primitives.
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+
+__device__ __forceinline__ uint64_t ld_acq(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ bool cas64(uint64_t* p, uint64_t& expected, uint64_t desired)
+{
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t add64(uint64_t* p, uint64_t v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void or64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void and64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void min64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void max64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void or32(uint32_t* p, uint32_t v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Masks of the 4 bytes of x equal to the bytes of cc, as bits 0..3.
+__device__ __forceinline__ uint32_t byte_eq4(uint32_t x, uint32_t cc)
+{
+    const uint32_t y = x ^ cc;
+    const uint32_t t = ~(((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y | 0x7F7F7F7Fu);
+    return ((t >> 7) * 0x01020408u) >> 24;
+}
+
+// The slots of a bucket whose checksums equal `c`, and the empty ones.
+__device__ __forceinline__ void cs_masks(const uint8_t* meta, uint8_t c, uint32_t& eq, uint32_t& zero)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(meta);
+    const uint4 a = p[0], b = p[1];
+    const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    const uint32_t cc = 0x01010101u * c;
+    eq = 0, zero = 0;
+#pragma unroll
+    for(int i = 0; i < 8; ++i)
+    {
+        eq |= byte_eq4(d[i], cc) << (4 * i);
+        zero |= byte_eq4(d[i], 0u) << (4 * i);
+    }
+}
+
+// A table as the kernels see it.
+struct TableView
+{
+    uint8_t* buckets;
+    uint64_t cap;           // buckets
+    uint64_t* ov;           // overflow slots
+    uint64_t ov_mask;       // overflow slots - 1
+    uint64_t vmax;          // largest value of the width
+    uint32_t width;         // value bytes (0 in a set)
+};
+
+template <bool MAP> struct Geo
+{
+    static constexpr uint32_t slot_words = MAP ? 2 : 1;                 // u64 words per slot
+    static constexpr uint32_t bucket_bytes = META_BYTES + B * 8 * slot_words;
+};
+
+template <bool MAP>
+__device__ __forceinline__ uint8_t* bucket_at(const TableView& T, uint64_t b) { return T.buckets + b * Geo<MAP>::bucket_bytes; }
+
+template <bool MAP>
+__device__ __forceinline__ uint64_t* slot_at(uint8_t* bucket, uint32_t j) { return reinterpret_cast<uint64_t*>(bucket + META_BYTES) + j * Geo<MAP>::slot_words; }
+
+enum Res : int { RES_FOUND = 0, RES_NEW = 1, RES_FULL = 2 };
+
+// Finds the stored key `sk` (checksum `c`) in the bucket, or claims its first
+// empty slot for it.  Slots fill in order and keys are never removed, so a
+// walk over the slots from the first empty one in order, claiming each empty
+// slot by CAS, finds a concurrent insertion of the same key wherever it went.
+// The checksums serve as hints only: they may lag their keys.
+template <bool MAP>
+__device__ __forceinline__ Res probe_insert(uint8_t* bucket, uint64_t sk, uint8_t c, uint64_t*& slot)
+{
+    uint32_t eq, zero;
+    cs_masks(bucket, c, eq, zero);
+    while(eq)
+    {
+        const uint32_t j = __builtin_ctz(eq);
+        eq &= eq - 1;
+        uint64_t* s = slot_at<MAP>(bucket, j);
+        if(ld_acq(s) == sk)
+            return slot = s, RES_FOUND;
+    }
+
+    for(uint32_t j = zero ? __builtin_ctz(zero) : B; j < B; ++j)
+    {
+        uint64_t* s = slot_at<MAP>(bucket, j);
+        uint64_t expected = 0;
+        if(cas64(s, expected, sk))
+        {
+            bucket[j] = c;
+            return slot = s, RES_NEW;
+        }
+        if(expected == sk)
+            return slot = s, RES_FOUND;
+    }
+
+    return RES_FULL;
+}
+
+// Read-only probe: the slot holding `sk`, or null; `full` iff the bucket's
+// last slot is taken (slots fill in order).
+template <bool MAP>
+__device__ __forceinline__ uint64_t* probe_find(uint8_t* bucket, uint64_t sk, uint8_t c, bool& full)
+{
+    uint32_t eq, zero;
+    cs_masks(bucket, c, eq, zero);
+    full = !(zero >> 31);
+    while(eq)
+    {
+        const uint32_t j = __builtin_ctz(eq);
+        eq &= eq - 1;
+        uint64_t* s = slot_at<MAP>(bucket, j);
+        if(*s == sk)
+            return s;
+    }
+    return nullptr;
+}
+
+// Overflow table: linear probing from XXH3_64bits(key) (include/Concurrent_Hash_Table.hpp:205-300).
+template <bool MAP>
+__device__ __forceinline__ Res ov_insert(const TableView& T, uint64_t key, uint64_t sk, uint64_t*& slot)
+{
+    uint64_t i = xxh3_8(key, 0) & T.ov_mask;
+    for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
+    {
+        uint64_t* s = T.ov + i * Geo<MAP>::slot_words;
+        uint64_t cur = ld_acq(s);
+        if(cur == sk)
+            return slot = s, RES_FOUND;
+        if(cur == 0)
+        {
+            if(cas64(s, cur, sk))
+                return slot = s, RES_NEW;
+            if(cur == sk)
+                return slot = s, RES_FOUND;
+        }
+    }
+    return RES_FULL;
+}
+
+template <bool MAP>
+__device__ __forceinline__ uint64_t* ov_find(const TableView& T, uint64_t key, uint64_t sk)
+{
+    uint64_t i = xxh3_8(key, 0) & T.ov_mask;
+    for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
+    {
+        uint64_t* s = T.ov + i * Geo<MAP>::slot_words;
+        const uint64_t cur = *s;
+        if(cur == sk)
+            return s;
+        if(cur == 0)
+            return nullptr;
+    }
+    return nullptr;
+}
+
+// Marks the spill of minimizer hash `h` in the bucket's filter (:1079-1085).
+__device__ __forceinline__ void mark_spill(uint8_t* bucket, uint64_t h)
+{
+    const uint32_t i = filter_pos(h);
+    or32(reinterpret_cast<uint32_t*>(bucket + 32) + (i >> 5), 1u << (i & 31));
+}
+
+__device__ __forceinline__ bool may_spill(const uint8_t* bucket, uint64_t h)
+{
+    const uint32_t i = filter_pos(h);
+    return (bucket[32 + (i >> 3)] >> (i & 7)) & 1u;
+}
+
+// Inserts with the reference's policy (include/Streaming_Kmer_Hash_Table.hpp:2431-2490):
+// the primary bucket; once it is full, the spill mark, then the secondaries
+// and the overflow table.  The secondaries go in a fixed order, p then q, so
+// that concurrent inserters of one key walk the same slots.  RES_FULL only if
+// the overflow table has no room.
+template <bool MAP>
+__device__ __forceinline__ Res insert_key(const TableView& T, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
+{
+    const uint64_t sk = ~key;
+    in_ov = false;
+    uint8_t* bb = bucket_at<MAP>(T, bucket_of(h, T.cap));
+    Res r = probe_insert<MAP>(bb, sk, c, slot);
+    if(r != RES_FULL)
+        return r;
+
+    mark_spill(bb, h);
+    uint64_t p, q;
+    secondaries(h, T.cap, p, q);
+    r = probe_insert<MAP>(bucket_at<MAP>(T, p), sk, c, slot);
+    if(r != RES_FULL)
+        return r;
+    r = probe_insert<MAP>(bucket_at<MAP>(T, q), sk, c, slot);
+    if(r != RES_FULL)
+        return r;
+
+    in_ov = true;
+    return ov_insert<MAP>(T, key, sk, slot);
+}
+
+// Lookup policy (include/Streaming_Kmer_Hash_Table.hpp:2548-2574): the
+// primary; absent unless it is full and its filter marks the minimizer; the
+// secondaries; the overflow table iff both are full.
+template <bool MAP>
+__device__ __forceinline__ uint64_t* find_key(const TableView& T, uint64_t key, uint64_t h, uint8_t c)
+{
+    const uint64_t sk = ~key;
+    uint8_t* bb = bucket_at<MAP>(T, bucket_of(h, T.cap));
+    bool full;
+    uint64_t* s = probe_find<MAP>(bb, sk, c, full);
+    if(s || !full || !may_spill(bb, h))
+        return s;
+
+    uint64_t p, q;
+    secondaries(h, T.cap, p, q);
+    bool full_p, full_q;
+    if((s = probe_find<MAP>(bucket_at<MAP>(T, p), sk, c, full_p)))
+        return s;
+    if((s = probe_find<MAP>(bucket_at<MAP>(T, q), sk, c, full_q)))
+        return s;
+    return (full_p && full_q) ? ov_find<MAP>(T, key, sk) : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Values.  An empty slot's value is 0; every operation here is one 64-bit
+// atomic (plus a rare fix-up), written so that the winner of a key's
+// insertion and concurrent updaters of it commute.
+
+// v + x saturating at vmax (capi/Kmer_Table.hpp:38): the sum, and a clamp by
+// whoever crosses vmax.  Narrow widths never wrap their 64-bit word.
+__device__ __forceinline__ void sat_add(uint64_t* v, uint64_t x, uint64_t vmax)
+{
+    const uint64_t old = add64(v, x);
+    if(old > vmax - x)
+    {
+        if(vmax == ~uint64_t(0))
+            __hip_atomic_exchange(v, vmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            min64(v, vmax);
+    }
+}
+
+// v + x modulo 2^(8 * width): whoever carries out of the width clears it.
+__device__ __forceinline__ void wrap_add(uint64_t* v, uint64_t x, uint64_t vmax)
+{
+    const uint64_t old = add64(v, x);
+    if(vmax != ~uint64_t(0) && (old & vmax) + x > vmax)
+        and64(v, vmax);
+}
+
+// Applies `op` to the value `v` with operand `x`; `winner` iff this call just
+// inserted the key, with the value 0 in place.
+template <uint32_t OP>
+__device__ __forceinline__ void value_op(uint64_t* v, uint64_t x, uint64_t vmax, bool winner)
+{
+    if constexpr(OP == OP_KEEP)
+    {
+        if(winner)
+            st_rel(v, x);
+    }
+    else if constexpr(OP == OP_ASSIGN)
+        st_rel(v, x);
+    else if constexpr(OP == OP_ADD)
+        sat_add(v, x, vmax);
+    else if constexpr(OP == OP_OR)
+        or64(v, x);
+    else if constexpr(OP == OP_MIN)
+    {
+        if(winner)
+            st_rel(v, x);
+        else
+            min64(v, x);
+    }
+    else if constexpr(OP == OP_MAX)
+        max64(v, x);
+    else
+        wrap_add(v, x, vmax);
+}
+
+// ---------------------------------------------------------------------------
+// The describer, per tile of TILE positions: each thread describes position
+// p0 + threadIdx.x.  Staging: 10 threads pack 320 characters into 2-bit
+// words and non-ACGT masks in LDS; every thread then builds its 32-base
+// window from two words, hashes its l-mer, and takes the minimum of 10
+// consecutive l-mer hashes from LDS (include/Kmer_Batch.hpp:191-250, the
+// same quantities per position).
+
+struct Desc
+{
+    uint64_t key;
+    uint64_t h;
+    uint32_t valid;
+    uint32_t fwd;
+    uint8_t c;
+};
+
+struct TileLds
+{
+    uint64_t pk[TILE_WORDS];
+    uint32_t bad[TILE_WORDS];
+    uint64_t lh[TILE + W];
+};
+
+__device__ __forceinline__ uint64_t window_at(const TileLds& s, int u)
+{
+    const int wi = u >> 5, sh = (u & 31) * 2;
+    const uint64_t a = s.pk[wi];
+    return sh ? (a << sh) | (s.pk[wi + 1] >> (64 - sh)) : a;
+}
+
+// `text + p0` must be 16-byte aligned with TILE_WORDS * 32 readable bytes.
+// Positions at or past `pos_end` are not described; a k-mer must also end
+// within the first `text_len` bytes.
+__device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, uint64_t pos_end, uint64_t text_len, TileLds& s)
+{
+    const int t = threadIdx.x;
+    if(t < TILE_WORDS)
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(text + p0 + 32 * t);
+        const uint4 a = src[0], b = src[1];
+        const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint64_t pk;
+        uint32_t bad;
+        pack32(d, pk, bad);
+        s.pk[t] = pk, s.bad[t] = bad;
+    }
+    __syncthreads();
+
+    Desc d;
+    uint64_t kf, kr, lh;
+    window_parts(window_at(s, t), kf, kr, lh);
+    s.lh[t] = lh;
+    if(t < W - 1)
+    {
+        uint64_t f2, r2, lh2;
+        window_parts(window_at(s, TILE + t), f2, r2, lh2);
+        s.lh[TILE + t] = lh2;
+    }
+
+    const int wi = t >> 5;
+    const uint64_t b64 = static_cast<uint64_t>(s.bad[wi]) | (static_cast<uint64_t>(s.bad[wi + 1]) << 32);
+    d.valid = (((b64 >> (t & 31)) & 0x7FFFFFFFull) == 0) && (p0 + t < pos_end) && (p0 + t + K <= text_len);
+    __syncthreads();
+
+    uint64_t h = s.lh[t];
+#pragma unroll
+    for(int o = 1; o < W; ++o)
+    {
+        const uint64_t y = s.lh[t + o];
+        h = y < h ? y : h;
+    }
+    d.h = h;
+    d.key = kf < kr ? kf : kr;
+    d.fwd = kf <= kr;
+    d.c = checksum(d.key);
+    return d;
+}
+
+// Sums `v` over the block into `*dst` with one atomic.
+__device__ __forceinline__ void block_add(uint64_t v, uint64_t* dst, uint64_t* red)
+{
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if(lane == 0)
+        red[wave] = v;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        uint64_t s = 0;
+        for(int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i)
+            s += red[i];
+        if(s)
+            add64(dst, s);
+    }
+    __syncthreads();
+}
+
+// Per-launch counters.
+struct Counters
+{
+    uint64_t inserted;      // keys new to the table
+    uint64_t ov_inserted;   // of which into the overflow table
+    uint64_t present;       // positions found (updates, queries)
+    uint64_t failed;        // positions that found no room (must stay 0)
+    uint64_t positions;     // k-mer positions seen (queries)
+};
+
+#endif
+
+}

@@ -1,0 +1,457 @@
+// Kernels of the MI355X k-mer table.  All are grid-stride ("persistent"):
+// the host launches about as many blocks as the device keeps resident, and
+// each block walks tiles until the range is done.
+#pragma once
+
+#include "kache_device.hpp"
+
+namespace kamd
+{
+
+// Sum over the wave.
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
+{
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Inserts / upserts (include/Streaming_Kmer_Hash_Table.hpp:2217-2514): every
+// valid position of [pos_begin, pos_end) of `text`.  `x` is the operand, or
+// the forward one with ORIENT (x_rev for k-mers occurring reverse-complemented).
+template <bool MAP, uint32_t OP, bool ORIENT>
+__global__ __launch_bounds__(TILE) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
+{
+    __shared__ TileLds s;
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t ins = 0, ins_ov = 0, failed = 0;
+    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
+    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
+        if(d.valid)
+        {
+            uint64_t* slot;
+            bool in_ov;
+            const Res r = insert_key<MAP>(T, d.key, d.h, d.c, slot, in_ov);
+            if(r == RES_FULL)
+                failed++;
+            else
+            {
+                ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
+                if constexpr(MAP)
+                    value_op<OP>(slot + 1, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
+            }
+        }
+    }
+    block_add(ins, &cnt->inserted, red);
+    block_add(ins_ov, &cnt->ov_inserted, red);
+    block_add(failed, &cnt->failed, red);
+}
+
+// Present-only updates (include/Streaming_Kmer_Hash_Table.hpp:2648-2691).
+template <uint32_t OP>
+__global__ __launch_bounds__(TILE) void k_update(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, Counters* cnt)
+{
+    __shared__ TileLds s;
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t present = 0;
+    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
+    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
+        if(d.valid)
+            if(uint64_t* slot = find_key<true>(T, d.key, d.h, d.c))
+            {
+                present++;
+                value_op<OP>(slot + 1, x, T.vmax, false);
+            }
+    }
+    block_add(present, &cnt->present, red);
+}
+
+// ---------------------------------------------------------------------------
+// Queries.  The staged text holds segments: segment s starts at seg_start[s],
+// belongs to sequence seg_q[s] and starts at its position seg_off[s];
+// tile_seg[t] is the segment holding position t * TILE.
+
+enum QueryMode : int { Q_TOTAL = 0, Q_COUNT = 1, Q_SUM = 2, Q_MASKED = 3, Q_LOOKUP = 4 };
+
+struct QueryArgs
+{
+    const uint64_t* seg_start;
+    const uint32_t* seg_q;
+    const uint64_t* seg_off;
+    const uint32_t* tile_seg;
+    const uint64_t* out_off;    // per sequence: index of its first position in the lookup outputs
+    uint64_t* per_seq;          // counts / sums / masked counts, per sequence
+    uint8_t* found;
+    uint8_t* values;
+    uint8_t* fwd;
+    uint32_t vbytes;
+    uint64_t mask_fwd, mask_rev;
+};
+
+// The segment holding position p, from the tile's bracket.
+__device__ __forceinline__ uint32_t segment_of(const QueryArgs& a, uint64_t tile, uint64_t p)
+{
+    uint32_t lo = a.tile_seg[tile], hi = a.tile_seg[tile + 1];
+    while(lo < hi)
+    {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if(a.seg_start[mid] <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool MAP, int MODE>
+__global__ __launch_bounds__(TILE) void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_end, QueryArgs a, Counters* cnt)
+{
+    __shared__ TileLds s;
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t total = 0, positions = 0;
+    const uint64_t tiles = (pos_end + TILE - 1) / TILE;
+    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const uint64_t p = tile * TILE + threadIdx.x;
+        const Desc d = describe_tile(text, tile * TILE, pos_end, pos_end, s);
+        uint64_t v = 0;
+        bool hit = false;
+        if(d.valid)
+            if(const uint64_t* slot = find_key<MAP>(T, d.key, d.h, d.c))
+                hit = true, v = MAP ? slot[1] : 0;
+
+        if constexpr(MODE == Q_TOTAL)
+            total += hit, positions += d.valid;
+        else if constexpr(MODE == Q_LOOKUP)
+        {
+            if(d.valid)
+            {
+                const uint32_t sg = segment_of(a, tile, p);
+                const uint64_t o = a.out_off[a.seg_q[sg]] + a.seg_off[sg] + (p - a.seg_start[sg]);
+                a.found[o] = hit ? 1 : 0;
+                if(a.fwd)
+                    a.fwd[o] = static_cast<uint8_t>(d.fwd);
+                if(a.values)
+                {
+                    if(a.vbytes == 1)
+                        a.values[o] = static_cast<uint8_t>(v);
+                    else if(a.vbytes == 2)
+                        reinterpret_cast<uint16_t*>(a.values)[o] = static_cast<uint16_t>(v);
+                    else if(a.vbytes == 4)
+                        reinterpret_cast<uint32_t*>(a.values)[o] = static_cast<uint32_t>(v);
+                    else
+                        reinterpret_cast<uint64_t*>(a.values)[o] = v;
+                }
+            }
+        }
+        else
+        {
+            // Per-sequence sums: lanes of one sequence add once per wave.
+            uint64_t add = 0;
+            if(hit)
+                add = MODE == Q_COUNT ? 1 : MODE == Q_SUM ? v : ((v & (d.fwd ? a.mask_fwd : a.mask_rev)) != 0);
+            uint32_t q = 0xFFFFFFFFu;
+            if(add)
+                q = a.seg_q[segment_of(a, tile, p)];
+            uint64_t pending = __ballot(add != 0);
+            while(pending)
+            {
+                const int leader = __builtin_ctzll(pending);
+                const uint32_t q0 = __shfl(q, leader, 64);
+                const bool mine = (add != 0) && (q == q0);
+                const uint64_t sum = wave_sum(mine ? add : 0);
+                if((threadIdx.x & 63) == leader)
+                    add64(a.per_seq + q0, sum);
+                pending &= ~__ballot(mine);
+            }
+        }
+    }
+    if constexpr(MODE == Q_TOTAL)
+    {
+        block_add(total, &cnt->present, red);
+        block_add(positions, &cnt->positions, red);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Resize: doubling into a new, zeroed table of 2 * cap buckets
+// (include/Streaming_Kmer_Hash_Table.hpp:1473-1681), out of place.
+// Phase 1: a key at its primary bucket b moves to 2b or 2b + 1, in its slot
+// order; half a wave per old bucket, so the order is a ballot prefix count.
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N)
+{
+    const uint32_t lane = threadIdx.x & 63, j = lane & 31;
+    const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    const uint64_t pairs = (O.cap + 1) / 2;
+    for(uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; w < pairs; w += (uint64_t(gridDim.x) * blockDim.x) >> 6)
+    {
+        const uint64_t b = 2 * w + (lane >> 5);
+        bool lo = false, hi = false;
+        uint64_t sk = 0, val = 0;
+        uint8_t c = 0;
+        if(b < O.cap)
+        {
+            uint8_t* ob = bucket_at<MAP>(O, b);
+            c = ob[j];
+            if(c)
+            {
+                const uint64_t* os = slot_at<MAP>(ob, j);
+                sk = os[0];
+                if(MAP)
+                    val = os[1];
+                const uint64_t h = minimizer_hash(~sk);
+                if(bucket_of(h, O.cap) == b)
+                {
+                    hi = (bucket_of(h, N.cap) == 2 * b + 1);
+                    lo = !hi;
+                }
+            }
+        }
+        const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask;
+        if(lo || hi)
+        {
+            const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
+            uint8_t* nb = bucket_at<MAP>(N, 2 * b + (hi ? 1 : 0));
+            uint64_t* ns = slot_at<MAP>(nb, r);
+            ns[0] = sk;
+            if(MAP)
+                ns[1] = val;
+            nb[r] = c;
+        }
+    }
+}
+
+// Phase 2: the keys away from their primary buckets, and the old overflow
+// table's, go in with the insertion policy, checksums and minimizers
+// recomputed (:1575-1665).
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_reinsert(TableView O, TableView N, Counters* cnt)
+{
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t ov_n = 0, failed = 0;
+    const uint64_t main_slots = O.cap * B, all = main_slots + O.ov_mask + 1;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
+    {
+        uint64_t sk, val = 0, h;
+        if(g < main_slots)
+        {
+            const uint64_t b = g / B;
+            const uint32_t j = g % B;
+            uint8_t* ob = bucket_at<MAP>(O, b);
+            if(!ob[j])
+                continue;
+            const uint64_t* os = slot_at<MAP>(ob, j);
+            sk = os[0];
+            if(MAP)
+                val = os[1];
+            h = minimizer_hash(~sk);
+            if(bucket_of(h, O.cap) == b)
+                continue;
+        }
+        else
+        {
+            const uint64_t* os = O.ov + (g - main_slots) * Geo<MAP>::slot_words;
+            sk = os[0];
+            if(!sk)
+                continue;
+            if(MAP)
+                val = os[1];
+            h = minimizer_hash(~sk);
+        }
+
+        uint64_t* slot;
+        bool in_ov;
+        const uint64_t key = ~sk;
+        const Res r = insert_key<MAP>(N, key, h, checksum(key), slot, in_ov);
+        if(r == RES_FULL)
+            failed++;
+        else
+        {
+            ov_n += in_ov;
+            if(MAP)
+                st_rel(slot + 1, val);
+        }
+    }
+    block_add(ov_n, &cnt->ov_inserted, red);
+    block_add(failed, &cnt->failed, red);
+}
+
+// Overflow growth: the old overflow table's keys into a new one
+// (include/Streaming_Kmer_Hash_Table.hpp:1412-1443).
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint64_t old_slots, TableView N, Counters* cnt)
+{
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t failed = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < old_slots; g += stride)
+    {
+        const uint64_t* os = old_ov + g * Geo<MAP>::slot_words;
+        const uint64_t sk = os[0];
+        if(!sk)
+            continue;
+        uint64_t* slot;
+        if(ov_insert<MAP>(N, ~sk, sk, slot) == RES_FULL)
+            failed++;
+        else if(MAP)
+            st_rel(slot + 1, os[1]);
+    }
+    block_add(failed, &cnt->failed, red);
+}
+
+// ---------------------------------------------------------------------------
+// Scans: iteration shares, reductions and placement statistics.
+
+// Copies the keys (2-bit encodings) and values of buckets [b0, b1) and
+// overflow slots [o0, o1) to out_keys / out_vals, at cursor positions.
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint64_t b1, uint64_t o0, uint64_t o1, uint64_t* out_keys, uint64_t* out_vals, uint64_t* cursor)
+{
+    const uint64_t main_n = (b1 - b0) * B, all = main_n + (o1 - o0);
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t base_g = blockIdx.x * uint64_t(blockDim.x) + (threadIdx.x & ~63u);
+    for(uint64_t g0 = base_g; g0 < all; g0 += stride)
+    {
+        const uint64_t g = g0 + (threadIdx.x & 63);
+        uint64_t sk = 0, val = 0;
+        if(g < main_n)
+        {
+            uint8_t* bk = bucket_at<MAP>(T, b0 + g / B);
+            if(bk[g % B])
+            {
+                const uint64_t* s = slot_at<MAP>(bk, g % B);
+                sk = s[0];
+                if(MAP)
+                    val = s[1];
+            }
+        }
+        else if(g < all)
+        {
+            const uint64_t* s = T.ov + (o0 + g - main_n) * Geo<MAP>::slot_words;
+            sk = s[0];
+            if(MAP && sk)
+                val = s[1];
+        }
+        const uint64_t m = __ballot(sk != 0);
+        if(!m)
+            continue;
+        const uint32_t lane = threadIdx.x & 63;
+        uint64_t base = 0;
+        if(lane == static_cast<uint32_t>(__builtin_ctzll(m)))
+            base = add64(cursor, __popcll(m));
+        base = __shfl(base, __builtin_ctzll(m), 64);
+        if(sk)
+        {
+            const uint64_t i = base + __popcll(m & ((uint64_t(1) << lane) - 1));
+            out_keys[i] = ~sk;
+            if(MAP)
+                out_vals[i] = val;
+        }
+    }
+}
+
+// capi/Kmer_Table.hpp:203-230: KEYS (v & m nonzero; every key of a set),
+// POPCOUNT, SUM of v & m.
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uint64_t mask, uint64_t* out)
+{
+    __shared__ uint64_t red[TILE / 64];
+    const uint64_t main_n = T.cap * B, all = main_n + T.ov_mask + 1;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t acc = 0;
+    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
+    {
+        const uint64_t* s;
+        if(g < main_n)
+        {
+            uint8_t* bk = bucket_at<MAP>(T, g / B);
+            if(!bk[g % B])
+                continue;
+            s = slot_at<MAP>(bk, g % B);
+        }
+        else
+        {
+            s = T.ov + (g - main_n) * Geo<MAP>::slot_words;
+            if(!s[0])
+                continue;
+        }
+        if(!MAP)
+        {
+            acc++;
+            continue;
+        }
+        const uint64_t v = s[1] & mask;
+        acc += what == 0 ? (v != 0) : what == 1 ? static_cast<uint64_t>(__popcll(v)) : v;
+    }
+    block_add(acc, out, red);
+}
+
+// Keys of the main table at their primary bucket, and elsewhere
+// (include/Streaming_Kmer_Hash_Table.hpp:1109-1120, 2728-2752).
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_placement(TableView T, uint64_t* out)
+{
+    __shared__ uint64_t red[TILE / 64];
+    const uint64_t all = T.cap * B, stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t prim = 0, sec = 0;
+    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
+    {
+        uint8_t* bk = bucket_at<MAP>(T, g / B);
+        if(!bk[g % B])
+            continue;
+        const uint64_t h = minimizer_hash(~slot_at<MAP>(bk, g % B)[0]);
+        if(bucket_of(h, T.cap) == g / B)
+            prim++;
+        else
+            sec++;
+    }
+    block_add(prim, out, red);
+    block_add(sec, out + 1, red);
+}
+
+__global__ void k_fill(uint8_t* p, uint64_t n, uint8_t v)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+// Zeroes n u64 words (16-byte aligned buffers, n even or odd).
+__global__ void k_zero64(uint64_t* p, uint64_t n)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += stride)
+        p[i] = 0;
+}
+
+// Describes the valid positions of [pos_begin, pos_end) into arrays (for
+// tests: the device describer checked position by position).
+__global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid)
+{
+    __shared__ TileLds s;
+    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
+    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const uint64_t p = pos_begin + tile * TILE + threadIdx.x;
+        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
+        if(p < pos_end)
+        {
+            const uint64_t i = p - pos_begin;
+            valid[i] = static_cast<uint8_t>(d.valid);
+            key[i] = d.valid ? d.key : 0;
+            h[i] = d.valid ? d.h : 0;
+            c[i] = d.valid ? d.c : 0;
+            fwd[i] = d.valid ? static_cast<uint8_t>(d.fwd) : 0;
+        }
+    }
+}
+
+}

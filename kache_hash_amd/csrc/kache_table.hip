@@ -1,0 +1,1382 @@
+// Host side of the MI355X k-mer table: the C ABI of include/kache_capi.h and
+// the device extensions of include/kache_amd.h.
+//
+// A table owns its HBM bucket store, its overflow table, a HIP stream and
+// staging buffers.  Each call stages its sequences (pinned host -> HBM),
+// launches the fused describe-and-probe kernels, and reads back results.
+// Growth follows the reference's schedule exactly (per-user flush steps,
+// include/Streaming_Kmer_Hash_Table.hpp:2244-2286), so capacities match it.
+
+#include "kache_capi.h"
+#include "kache_amd.h"
+#include "kache_kernels.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace kamd;
+
+namespace
+{
+
+thread_local std::string last_error;
+
+struct hip_failure : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what)
+{
+    if(e == hipSuccess)
+        return;
+    if(e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    {
+        last_error = std::string(what) + ": out of device memory";
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    throw hip_failure(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP(call) check((call), #call)
+
+// Runs `f`, returning its status, or that of the exception it throws
+// (capi/kache_capi.cpp:14-29).
+template <typename F>
+int guard(const F& f) noexcept
+{
+    try
+    {
+        return f();
+    }
+    catch(const std::bad_alloc&)
+    {
+        if(last_error.empty())
+            last_error = "out of memory";
+        return KACHE_ENOMEM;
+    }
+    catch(const std::exception& e)
+    {
+        last_error = e.what();
+        return KACHE_EINTERNAL;
+    }
+    catch(...)
+    {
+        last_error = "unknown error";
+        return KACHE_EINTERNAL;
+    }
+}
+
+constexpr uint64_t MAX_CAP = (uint64_t(1) << 32) - 1;   // include/Streaming_Kmer_Hash_Table.hpp:84
+constexpr uint64_t OV_MIN = 1024;                       // :67
+constexpr uint64_t LAUNCH_POS = uint64_t(1) << 24;      // positions per insert launch
+constexpr uint64_t STAGE_BYTES = uint64_t(1) << 28;     // host sequence bytes staged at once
+constexpr uint64_t COLLECT_BUCKETS = uint64_t(1) << 18; // buckets per iteration copy-out
+
+uint64_t ceil_pow2(uint64_t x)
+{
+    uint64_t p = 1;
+    while(p < x)
+        p <<= 1;
+    return p;
+}
+
+// include/Streaming_Kmer_Hash_Table.hpp:1003-1011.
+uint64_t cap_for(uint64_t n, double lf, bool pow2)
+{
+    uint64_t cap = std::max<uint64_t>(1, static_cast<uint64_t>(std::ceil(static_cast<double>(n) / lf / B)));
+    while(static_cast<uint64_t>(static_cast<double>(cap * B) * lf) <= n)
+        cap++;
+    return pow2 ? ceil_pow2(cap) : cap;
+}
+
+// :799-804.
+bool size_fits(uint64_t n, double lf, bool pow2)
+{
+    return static_cast<double>(n) / lf / B <= static_cast<double>(MAX_CAP) && cap_for(n, lf, pow2) <= MAX_CAP;
+}
+
+// :383.
+uint64_t step_for(uint64_t th) { return std::clamp<uint64_t>(th / (8 * KACHE_MAX_USERS), 1, 16384); }
+
+// Slots of an overflow table with room for n keys at load 0.8
+// (include/Concurrent_Hash_Table.hpp:161).
+uint64_t ov_slots_for(uint64_t n) { return ceil_pow2(static_cast<uint64_t>(std::ceil(static_cast<double>(n) / 0.8))); }
+
+template <typename T>
+T* dev_alloc(uint64_t n)
+{
+    void* p = nullptr;
+    if(n == 0)
+        n = 1;
+    HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+int device_cus()
+{
+    static int cus = [](){
+        int dev = 0, n = 0;
+        if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return n > 0 ? n : 256;
+    }();
+    return cus;
+}
+
+// A growable device buffer.
+struct DevBuf
+{
+    void* p = nullptr;
+    uint64_t bytes = 0;
+    void reserve(uint64_t n)
+    {
+        if(n <= bytes)
+            return;
+        if(p)
+            HIP(hipFree(p)), p = nullptr, bytes = 0;
+        n = std::max<uint64_t>(n, 4096);
+        HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    ~DevBuf() { if(p) (void)hipFree(p); }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// A growable pinned host buffer.
+struct HostBuf
+{
+    void* p = nullptr;
+    uint64_t bytes = 0;
+    void reserve(uint64_t n)
+    {
+        if(n <= bytes)
+            return;
+        if(p)
+            HIP(hipHostFree(p)), p = nullptr, bytes = 0;
+        n = std::max<uint64_t>(n, 4096);
+        HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+        bytes = n;
+    }
+    ~HostBuf() { if(p) (void)hipHostFree(p); }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Sequences staged for one device pass: segments of the caller's sequences,
+// each followed by a separator byte (include/Streaming_Kmer_Hash_Table.hpp:1697-1709
+// splits at non-ACGT characters, so the separator splits too).
+struct Stage
+{
+    uint64_t text_len = 0;
+    std::vector<uint64_t> seg_start;    // n_seg + 1
+    std::vector<uint32_t> seg_q;
+    std::vector<uint64_t> seg_off;
+};
+
+// Cursor over the caller's sequences: sequence q from its character `off`.
+struct SeqCursor
+{
+    size_t q = 0;
+    uint64_t off = 0;
+};
+
+}
+
+
+// ---------------------------------------------------------------------------
+// The table.
+
+struct kache_table
+{
+    const uint32_t value_bytes;
+    std::mutex reg;                         // serializes registrations (capi/kache_table.hpp:22-23)
+    std::atomic<uint32_t> user_c{0};
+
+    mutable std::mutex mu;                  // serializes device work
+    hipStream_t stream = nullptr;
+
+    const double lf;
+    const bool map;
+
+    uint64_t cap = 0;                       // buckets
+    uint8_t* buckets = nullptr;
+    uint64_t* ov = nullptr;
+    uint64_t ov_slots = 0;
+    uint64_t ov_count = 0;
+    uint64_t size = 0;                      // keys
+    uint64_t resizes = 0;
+
+    // The reference's growth schedule (include/Streaming_Kmer_Hash_Table.hpp:224-237).
+    uint64_t approx_sz = 0;
+    uint64_t resize_th = 0;
+    uint64_t step = 1;
+    uint64_t checkpoint[KACHE_MAX_USERS] = {};
+
+    // Staging and scratch.
+    mutable HostBuf h_text, h_misc;
+    mutable DevBuf d_text, d_misc, d_out, d_out2, d_out3;
+    mutable Counters* d_cnt = nullptr;
+    mutable Counters* h_cnt = nullptr;
+    uint32_t blocks_per_cu = 0;
+
+    // Profiling.
+    bool profiling = false;
+    mutable std::map<std::string, std::pair<double, uint64_t>> prof;
+
+    kache_table(const kache_options& o)
+        : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0)
+    {
+        HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        d_cnt = dev_alloc<Counters>(1);
+        HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(Counters), hipHostMallocDefault));
+        try
+        {
+            cap = cap_for(o.expected_keys, lf, o.pow2_capacity != 0);
+            resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
+            step = step_for(resize_th);
+            buckets = alloc_buckets(cap);
+            ov_slots = ov_slots_for(std::max<uint64_t>(OV_MIN, cap));
+            ov = alloc_ov(ov_slots);
+            HIP(hipStreamSynchronize(stream));
+        }
+        catch(...)
+        {
+            release();
+            throw;
+        }
+    }
+
+    ~kache_table() { release(); }
+
+    void release()
+    {
+        if(stream)
+            (void)hipStreamSynchronize(stream);
+        if(buckets)
+            (void)hipFree(buckets), buckets = nullptr;
+        if(ov)
+            (void)hipFree(ov), ov = nullptr;
+        if(d_cnt)
+            (void)hipFree(d_cnt), d_cnt = nullptr;
+        if(h_cnt)
+            (void)hipHostFree(h_cnt), h_cnt = nullptr;
+        if(stream)
+            (void)hipStreamDestroy(stream), stream = nullptr;
+    }
+
+    uint32_t bucket_bytes() const { return map ? Geo<true>::bucket_bytes : Geo<false>::bucket_bytes; }
+    uint32_t slot_words() const { return map ? 2 : 1; }
+    uint64_t vmax() const { return value_bytes == 8 ? ~uint64_t(0) : value_bytes == 0 ? 0 : (uint64_t(1) << (8 * value_bytes)) - 1; }
+
+    uint8_t* alloc_buckets(uint64_t n) const
+    {
+        uint8_t* p = dev_alloc<uint8_t>(n * bucket_bytes());
+        zero(p, n * bucket_bytes());
+        return p;
+    }
+
+    uint64_t* alloc_ov(uint64_t slots) const
+    {
+        uint64_t* p = dev_alloc<uint64_t>(slots * slot_words());
+        zero(p, slots * slot_words() * 8);
+        return p;
+    }
+
+    void zero(void* p, uint64_t bytes) const { HIP(hipMemsetAsync(p, 0, bytes, stream)); }
+
+    TableView view() const { return view_of(buckets, cap, ov, ov_slots); }
+
+    TableView view_of(uint8_t* b, uint64_t c, uint64_t* o, uint64_t os) const
+    {
+        TableView v;
+        v.buckets = b, v.cap = c, v.ov = o, v.ov_mask = os - 1, v.vmax = vmax(), v.width = value_bytes;
+        return v;
+    }
+
+    // Blocks of a persistent launch: as many as stay resident, capped by the work.
+    template <typename Kern>
+    uint32_t grid_for(Kern kern, uint64_t work_blocks) const
+    {
+        int per_cu = static_cast<int>(blocks_per_cu);
+        if(per_cu == 0)
+        {
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TILE, 0) != hipSuccess || per_cu < 1)
+                per_cu = 4;
+        }
+        const uint64_t g = static_cast<uint64_t>(per_cu) * device_cus();
+        return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, work_blocks)));
+    }
+
+    // Launches `launch()` and, when profiling, brackets it with events.
+    template <typename F>
+    void timed(const char* name, const F& launch) const
+    {
+        if(!profiling)
+        {
+            launch();
+            HIP(hipGetLastError());
+            return;
+        }
+        hipEvent_t a, b;
+        HIP(hipEventCreate(&a));
+        HIP(hipEventCreate(&b));
+        HIP(hipEventRecord(a, stream));
+        launch();
+        HIP(hipGetLastError());
+        HIP(hipEventRecord(b, stream));
+        HIP(hipEventSynchronize(b));
+        float ms = 0;
+        HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a), (void)hipEventDestroy(b);
+        auto& e = prof[name];
+        e.first += ms, e.second++;
+    }
+
+    void zero_counters() const
+    {
+        hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), sizeof(Counters) / 8);
+        HIP(hipGetLastError());
+    }
+
+    Counters read_counters() const
+    {
+        HIP(hipMemcpyAsync(h_cnt, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+        HIP(hipStreamSynchronize(stream));
+        return *h_cnt;
+    }
+
+    // --- Overflow growth (include/Streaming_Kmer_Hash_Table.hpp:1412-1443) ---
+
+    void rehash_ov(uint64_t new_slots)
+    {
+        uint64_t* nov = alloc_ov(new_slots);
+        const TableView N = view_of(buckets, cap, nov, new_slots);
+        zero_counters();
+        const uint64_t blocks = (ov_slots + TILE - 1) / TILE;
+        timed("ov_rehash", [&]{
+            if(map)
+                hipLaunchKernelGGL(k_ov_rehash<true>, dim3(grid_for(k_ov_rehash<true>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+            else
+                hipLaunchKernelGGL(k_ov_rehash<false>, dim3(grid_for(k_ov_rehash<false>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+        });
+        const Counters c = read_counters();
+        if(c.failed)
+        {
+            (void)hipFree(nov);
+            throw hip_failure("overflow rehash found no room");
+        }
+        HIP(hipFree(ov));
+        ov = nov, ov_slots = new_slots;
+    }
+
+    // Grows the overflow table until it has room for `incoming` more keys
+    // without passing the reference's load (0.75, :66) by more than them.
+    void ov_room(uint64_t incoming)
+    {
+        uint64_t slots = ov_slots;
+        while(ov_count + incoming + 1 > slots || static_cast<double>(ov_count) >= 0.75 * static_cast<double>(slots))
+            slots *= 2;
+        if(slots != ov_slots)
+            rehash_ov(slots);
+    }
+
+    // --- Resize: doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681) ---
+
+    void resize()
+    {
+        if(2 * cap > MAX_CAP)
+            throw hip_failure("growing the table past the most buckets supported");
+
+        const uint64_t ncap = 2 * cap;
+        // :1387-1392: the new overflow table takes the old one's keys' room, at least.
+        const uint64_t nov_slots = std::max<uint64_t>(ov_slots_for(std::max<uint64_t>(OV_MIN, ncap)), ov_slots);
+        uint8_t* nb = alloc_buckets(ncap);
+        uint64_t* nov = nullptr;
+        try
+        {
+            nov = alloc_ov(nov_slots);
+        }
+        catch(...)
+        {
+            (void)hipFree(nb);
+            throw;
+        }
+
+        const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
+        zero_counters();
+        const uint64_t split_blocks = (cap * 32 + TILE - 1) / TILE;
+        const uint64_t re_blocks = (cap * B + ov_slots + TILE - 1) / TILE;
+        timed("split", [&]{
+            if(map)
+                hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N);
+            else
+                hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N);
+        });
+        timed("reinsert", [&]{
+            if(map)
+                hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, O, N, d_cnt);
+            else
+                hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, O, N, d_cnt);
+        });
+        const Counters c = read_counters();
+        if(c.failed)
+            throw hip_failure("resize found no room in the overflow table");
+
+        HIP(hipFree(buckets));
+        HIP(hipFree(ov));
+        buckets = nb, cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = c.ov_inserted;
+        resizes++;
+        resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
+        step = step_for(resize_th);
+    }
+
+    // The growth schedule of one insert call by one user: the keys it inserts
+    // are flushed into the table's size every `step` keys, and a flush that
+    // reaches the threshold doubles the table
+    // (include/Streaming_Kmer_Hash_Table.hpp:2246-2286, 2509-2513).
+    struct CallSched
+    {
+        uint64_t pending, flushed = 0, ins_c = 0, due;
+    };
+
+    CallSched sched_begin(uint32_t user) const
+    {
+        CallSched s;
+        s.pending = checkpoint[user];
+        s.due = s.pending < step ? step - s.pending : 1;
+        return s;
+    }
+
+    // Accounts for `n` more keys inserted; returns the number of doublings due.
+    uint32_t sched_add(CallSched& s, uint64_t n)
+    {
+        uint32_t doublings = 0;
+        const uint64_t end = s.ins_c + n;
+        while(s.due <= end)
+        {
+            s.ins_c = s.due;
+            approx_sz += s.pending + (s.ins_c - s.flushed);
+            s.pending = 0, s.flushed = s.ins_c;
+            s.due = s.ins_c + step;
+            if(approx_sz >= resize_th)
+            {
+                doublings++;
+                // The threshold and step of the doubled table govern later flushes.
+                const uint64_t c2 = cap << doublings;
+                resize_th = static_cast<uint64_t>(static_cast<double>(c2 * B) * lf);
+                step = step_for(resize_th);
+            }
+        }
+        s.ins_c = end;
+        return doublings;
+    }
+
+    void sched_end(const CallSched& s, uint32_t user) { checkpoint[user] = s.pending + (s.ins_c - s.flushed); }
+
+    void grow(uint32_t doublings)
+    {
+        // sched_add already moved the thresholds; resize() recomputes them for
+        // the final capacity, which is the same.
+        const uint64_t th = resize_th, st = step;
+        for(uint32_t i = 0; i < doublings; ++i)
+            resize();
+        resize_th = th, step = st;
+    }
+
+    // --- Staging of host sequences ---
+
+    // Packs the caller's sequences from `cur` into the pinned staging buffer,
+    // up to about STAGE_BYTES, splitting a sequence too long for one stage
+    // into pieces that overlap by k - 1 characters.  Returns false when done.
+    bool stage_next(const char* const* seqs, const size_t* lens, size_t n, SeqCursor& cur, Stage& st) const
+    {
+        st.text_len = 0, st.seg_start.clear(), st.seg_q.clear(), st.seg_off.clear();
+        while(cur.q < n && (lens[cur.q] < static_cast<size_t>(K) || cur.off + K > lens[cur.q]))
+            cur.q++, cur.off = 0;
+        if(cur.q >= n)
+            return false;
+
+        // Size the stage.
+        uint64_t need = 0;
+        for(size_t q = cur.q; q < n && need < STAGE_BYTES; ++q)
+            need += (q == cur.q ? lens[q] - cur.off : lens[q]) + 1;
+        need = std::min<uint64_t>(need, STAGE_BYTES + K);
+        h_text.reserve(need + TEXT_PAD);
+        char* out = h_text.as<char>();
+
+        while(cur.q < n)
+        {
+            const size_t q = cur.q;
+            if(lens[q] < static_cast<size_t>(K) || cur.off + K > lens[q])
+            {
+                cur.q++, cur.off = 0;
+                continue;
+            }
+            const uint64_t room = need - st.text_len;
+            const uint64_t rest = lens[q] - cur.off;
+            if(rest + 1 <= room)
+            {
+                st.seg_start.push_back(st.text_len), st.seg_q.push_back(static_cast<uint32_t>(q)), st.seg_off.push_back(cur.off);
+                std::memcpy(out + st.text_len, seqs[q] + cur.off, rest);
+                st.text_len += rest;
+                out[st.text_len++] = '\n';
+                cur.q++, cur.off = 0;
+                continue;
+            }
+            if(room >= static_cast<uint64_t>(K) + 1 && (st.seg_start.empty() || room > (1u << 20)))
+            {
+                // A piece of the sequence; the next piece starts k - 1 characters before its end.
+                const uint64_t take = room - 1;
+                st.seg_start.push_back(st.text_len), st.seg_q.push_back(static_cast<uint32_t>(q)), st.seg_off.push_back(cur.off);
+                std::memcpy(out + st.text_len, seqs[q] + cur.off, take);
+                st.text_len += take;
+                out[st.text_len++] = '\n';
+                cur.off += take - (K - 1);
+            }
+            break;
+        }
+        std::memset(out + st.text_len, '\n', TEXT_PAD);
+        st.seg_start.push_back(st.text_len);
+        return true;
+    }
+
+    // Copies the staged text to the device.
+    const uint8_t* upload(const Stage& st) const
+    {
+        d_text.reserve(st.text_len + TEXT_PAD);
+        HIP(hipMemcpyAsync(d_text.p, h_text.p, st.text_len + TEXT_PAD, hipMemcpyHostToDevice, stream));
+        return d_text.as<uint8_t>();
+    }
+
+    // --- Inserts ---
+
+    template <bool MAP, uint32_t OP, bool OR>
+    void launch_insert(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
+    {
+        const TableView T = view();
+        const uint64_t tiles = (e - b + TILE - 1) / TILE;
+        hipLaunchKernelGGL((k_insert<MAP, OP, OR>), dim3(grid_for(k_insert<MAP, OP, OR>, tiles)), dim3(TILE), 0, stream, T, text, text_len, b, e, x, xr, d_cnt);
+    }
+
+    void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
+    {
+        if(!map)
+            return launch_insert<false, OP_KEEP, false>(text, text_len, b, e, 0, 0);
+        if(oriented)
+            switch(op)
+            {
+            case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, true>(text, text_len, b, e, x, xr);
+            case OP_ADD:    return launch_insert<true, OP_ADD, true>(text, text_len, b, e, x, xr);
+            case OP_OR:     return launch_insert<true, OP_OR, true>(text, text_len, b, e, x, xr);
+            default:        return launch_insert<true, OP_MAX, true>(text, text_len, b, e, x, xr);
+            }
+        switch(op)
+        {
+        case OP_KEEP:   return launch_insert<true, OP_KEEP, false>(text, text_len, b, e, x, 0);
+        case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, false>(text, text_len, b, e, x, 0);
+        case OP_ADD:    return launch_insert<true, OP_ADD, false>(text, text_len, b, e, x, 0);
+        case OP_OR:     return launch_insert<true, OP_OR, false>(text, text_len, b, e, x, 0);
+        case OP_MIN:    return launch_insert<true, OP_MIN, false>(text, text_len, b, e, x, 0);
+        case OP_MAX:    return launch_insert<true, OP_MAX, false>(text, text_len, b, e, x, 0);
+        default:        return launch_insert<true, OP_ADD_WRAP, false>(text, text_len, b, e, x, 0);
+        }
+    }
+
+    // Inserts the positions of a device text, in launches of at most
+    // LAUNCH_POS positions, growing the overflow table ahead of each so it
+    // cannot fill, and the table after each as the schedule says.
+    uint64_t insert_text(CallSched& s, uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t x, uint64_t xr)
+    {
+        uint64_t inserted = 0;
+        for(uint64_t b = 0; b < text_len; b += LAUNCH_POS)
+        {
+            const uint64_t e = std::min(text_len, b + LAUNCH_POS);
+            ov_room(e - b);
+            zero_counters();
+            timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
+            const Counters c = read_counters();
+            if(c.failed)
+                throw hip_failure("an insert found no room in the overflow table");
+            inserted += c.inserted, size += c.inserted, ov_count += c.ov_inserted;
+            grow(sched_add(s, c.inserted));
+        }
+        return inserted;
+    }
+
+    uint64_t insert_host(uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, bool oriented, uint64_t x, uint64_t xr)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        CallSched s = sched_begin(user);
+        uint64_t inserted = 0;
+        SeqCursor cur;
+        Stage st;
+        try
+        {
+            while(stage_next(seqs, lens, n, cur, st))
+                inserted += insert_text(s, op, oriented, upload(st), st.text_len, x, xr);
+        }
+        catch(...)
+        {
+            sched_end(s, user);
+            throw;
+        }
+        sched_end(s, user);
+        return inserted;
+    }
+
+    uint64_t insert_device(uint32_t user, const uint8_t* text, uint64_t text_len, uint32_t op, bool oriented, uint64_t x, uint64_t xr)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        HIP(hipDeviceSynchronize());
+        CallSched s = sched_begin(user);
+        uint64_t inserted = 0;
+        try
+        {
+            inserted = insert_text(s, op, oriented, text, text_len, x, xr);
+        }
+        catch(...)
+        {
+            sched_end(s, user);
+            throw;
+        }
+        sched_end(s, user);
+        return inserted;
+    }
+
+    // --- Updates ---
+
+    template <uint32_t OP>
+    void launch_update(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x) const
+    {
+        const uint64_t tiles = (e - b + TILE - 1) / TILE;
+        hipLaunchKernelGGL(k_update<OP>, dim3(grid_for(k_update<OP>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, b, e, x, d_cnt);
+    }
+
+    uint64_t update_text(uint32_t op, const uint8_t* text, uint64_t text_len, uint64_t x) const
+    {
+        zero_counters();
+        timed("update", [&]{
+            switch(op)
+            {
+            case OP_KEEP:   launch_update<OP_KEEP>(text, text_len, 0, text_len, x); break;
+            case OP_ASSIGN: launch_update<OP_ASSIGN>(text, text_len, 0, text_len, x); break;
+            case OP_ADD:    launch_update<OP_ADD>(text, text_len, 0, text_len, x); break;
+            case OP_OR:     launch_update<OP_OR>(text, text_len, 0, text_len, x); break;
+            case OP_MIN:    launch_update<OP_MIN>(text, text_len, 0, text_len, x); break;
+            case OP_MAX:    launch_update<OP_MAX>(text, text_len, 0, text_len, x); break;
+            default:        launch_update<OP_ADD_WRAP>(text, text_len, 0, text_len, x); break;
+            }
+        });
+        return read_counters().present;
+    }
+
+    uint64_t update_host(const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        uint64_t present = 0;
+        SeqCursor cur;
+        Stage st;
+        while(stage_next(seqs, lens, n, cur, st))
+            present += update_text(op, upload(st), st.text_len, x);
+        return present;
+    }
+
+    // --- Queries ---
+
+    template <int MODE>
+    void launch_query(const uint8_t* text, uint64_t text_len, const QueryArgs& a) const
+    {
+        const uint64_t tiles = (text_len + TILE - 1) / TILE;
+        timed("query", [&]{
+            if(map)
+                hipLaunchKernelGGL((k_query<true, MODE>), dim3(grid_for(k_query<true, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, a, d_cnt);
+            else
+                hipLaunchKernelGGL((k_query<false, MODE>), dim3(grid_for(k_query<false, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, a, d_cnt);
+        });
+    }
+
+    // Uploads the segment tables of a stage and fills `a` with them.
+    void upload_segments(const Stage& st, QueryArgs& a) const
+    {
+        const uint64_t ns = st.seg_q.size();
+        const uint64_t tiles = (st.text_len + TILE - 1) / TILE;
+        // Layout in one pinned block: seg_start (ns+1), seg_off (ns), seg_q (ns), tile_seg (tiles+1).
+        const uint64_t bytes = (ns + 1) * 8 + ns * 8 + ns * 4 + (tiles + 2) * 4 + 64;
+        h_misc.reserve(bytes);
+        d_misc.reserve(bytes);
+        uint8_t* h = h_misc.as<uint8_t>();
+        uint64_t* h_start = reinterpret_cast<uint64_t*>(h);
+        uint64_t* h_off = h_start + ns + 1;
+        uint32_t* h_q = reinterpret_cast<uint32_t*>(h_off + ns);
+        uint32_t* h_tile = h_q + ns;
+        std::memcpy(h_start, st.seg_start.data(), (ns + 1) * 8);
+        std::memcpy(h_off, st.seg_off.data(), ns * 8);
+        std::memcpy(h_q, st.seg_q.data(), ns * 4);
+        uint32_t sg = 0;
+        for(uint64_t t = 0; t <= tiles; ++t)
+        {
+            const uint64_t p = t * TILE;
+            while(sg + 1 < ns && st.seg_start[sg + 1] <= p)
+                sg++;
+            h_tile[t] = sg;
+        }
+        HIP(hipMemcpyAsync(d_misc.p, h, bytes, hipMemcpyHostToDevice, stream));
+        uint8_t* d = d_misc.as<uint8_t>();
+        a.seg_start = reinterpret_cast<const uint64_t*>(d);
+        a.seg_off = a.seg_start + ns + 1;
+        a.seg_q = reinterpret_cast<const uint32_t*>(a.seg_off + ns);
+        a.tile_seg = a.seg_q + ns;
+    }
+
+    // Per-sequence counts, sums or masked counts (capi/Kmer_Table.hpp:134-161).
+    template <int MODE>
+    void per_sequence(const char* const* seqs, const size_t* lens, size_t n, uint64_t* out, uint64_t mf, uint64_t mr) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        std::fill(out, out + n, 0);
+        if(n == 0)
+            return;
+        d_out.reserve(n * 8);
+        HIP(hipMemsetAsync(d_out.p, 0, n * 8, stream));
+        SeqCursor cur;
+        Stage st;
+        bool any = false;
+        while(stage_next(seqs, lens, n, cur, st))
+        {
+            any = true;
+            const uint8_t* text = upload(st);
+            QueryArgs a{};
+            upload_segments(st, a);
+            a.per_seq = d_out.as<uint64_t>();
+            a.mask_fwd = mf, a.mask_rev = mr;
+            launch_query<MODE>(text, st.text_len, a);
+            HIP(hipStreamSynchronize(stream));  // the staging buffers are reused
+        }
+        if(any)
+        {
+            HIP(hipMemcpyAsync(out, d_out.p, n * 8, hipMemcpyDeviceToHost, stream));
+            HIP(hipStreamSynchronize(stream));
+        }
+    }
+
+    // capi/Kmer_Table.hpp:163-190.
+    void lookup(const char* const* seqs, const size_t* lens, size_t n, uint8_t* found, void* values, uint8_t* fwd) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        std::vector<uint64_t> o(n + 1);
+        uint64_t p = 0;
+        for(size_t i = 0; i < n; ++i)
+            o[i] = p, p += (lens[i] >= static_cast<size_t>(K) ? lens[i] - K + 1 : 0);
+        o[n] = p;
+        if(p == 0)
+            return;
+
+        const uint32_t vb = value_bytes;
+        d_out.reserve(p);                       // found
+        d_out2.reserve(values ? p * vb : 1);    // values
+        d_out3.reserve(fwd ? p : 1);            // fwd
+        DevBuf d_off;
+        d_off.reserve((n + 1) * 8);
+        HIP(hipMemcpyAsync(d_off.p, o.data(), (n + 1) * 8, hipMemcpyHostToDevice, stream));
+        fill(d_out.p, p, KACHE_NOT_KMER);
+        if(values)
+            fill(d_out2.p, p * vb, 0);
+        if(fwd)
+            fill(d_out3.p, p, 0);
+
+        SeqCursor cur;
+        Stage st;
+        while(stage_next(seqs, lens, n, cur, st))
+        {
+            const uint8_t* text = upload(st);
+            QueryArgs a{};
+            upload_segments(st, a);
+            a.out_off = d_off.as<uint64_t>();
+            a.found = d_out.as<uint8_t>();
+            a.values = values ? d_out2.as<uint8_t>() : nullptr;
+            a.fwd = fwd ? d_out3.as<uint8_t>() : nullptr;
+            a.vbytes = vb;
+            launch_query<Q_LOOKUP>(text, st.text_len, a);
+            HIP(hipStreamSynchronize(stream));
+        }
+        HIP(hipMemcpyAsync(found, d_out.p, p, hipMemcpyDeviceToHost, stream));
+        if(values)
+            HIP(hipMemcpyAsync(values, d_out2.p, p * vb, hipMemcpyDeviceToHost, stream));
+        if(fwd)
+            HIP(hipMemcpyAsync(fwd, d_out3.p, p, hipMemcpyDeviceToHost, stream));
+        HIP(hipStreamSynchronize(stream));
+    }
+
+    void fill(void* p, uint64_t n, uint8_t v) const
+    {
+        const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((n + TILE - 1) / TILE, 4096));
+        hipLaunchKernelGGL(k_fill, dim3(std::max<uint32_t>(g, 1)), dim3(TILE), 0, stream, static_cast<uint8_t*>(p), n, v);
+        HIP(hipGetLastError());
+    }
+
+    Counters count_device(const uint8_t* text, uint64_t text_len) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        HIP(hipDeviceSynchronize());
+        zero_counters();
+        QueryArgs a{};
+        launch_query<Q_TOTAL>(text, text_len, a);
+        return read_counters();
+    }
+
+    // --- Scans ---
+
+    // capi/Kmer_Table.hpp:232-268 over include/Streaming_Kmer_Hash_Table.hpp:2703-2725:
+    // share `shard` of `shards` is buckets [cap*w/W, cap*(w+1)/W) and its
+    // share of the overflow slots, handed out in chunks of at most 4096 keys.
+    void for_each(uint32_t shards, uint32_t shard, kache_keys_fn f, void* ctx) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        const uint64_t b0 = cap * shard / shards, b1 = cap * (shard + 1) / shards;
+        const uint64_t range = (ov_slots + shards - 1) / shards;
+        const uint64_t o0 = std::min<uint64_t>(shard * range, ov_slots), o1 = std::min<uint64_t>((shard + 1) * range, ov_slots);
+
+        constexpr size_t chunk = 4096;
+        std::vector<uint64_t> K_(chunk), Vw;
+        std::vector<uint8_t> Vb(map ? chunk * value_bytes : 0);
+        bool done = false;
+        std::vector<uint64_t> hk, hv;
+
+        const auto deliver = [&](uint64_t n)
+        {
+            for(uint64_t i = 0; i < n && !done; i += chunk)
+            {
+                const uint64_t c = std::min<uint64_t>(chunk, n - i);
+                std::memcpy(K_.data(), hk.data() + i, c * 8);
+                if(map)
+                    for(uint64_t j = 0; j < c; ++j)
+                        std::memcpy(Vb.data() + j * value_bytes, &hv[i + j], value_bytes);   // little-endian low bytes
+                done = (f(ctx, K_.data(), map ? Vb.data() : nullptr, c) != 0);
+            }
+        };
+
+        // Pieces of the share: bucket ranges, then the overflow range.
+        struct Piece { uint64_t b0, b1, o0, o1; };
+        std::vector<Piece> pieces;
+        for(uint64_t b = b0; b < b1; b += COLLECT_BUCKETS)
+            pieces.push_back({b, std::min(b1, b + COLLECT_BUCKETS), 0, 0});
+        for(uint64_t o = o0; o < o1; o += COLLECT_BUCKETS * B)
+            pieces.push_back({0, 0, o, std::min(o1, o + COLLECT_BUCKETS * B)});
+
+        DevBuf dk, dv, dc;
+        dk.reserve(COLLECT_BUCKETS * B * 8);
+        if(map)
+            dv.reserve(COLLECT_BUCKETS * B * 8);
+        dc.reserve(8);
+        for(const auto& pc : pieces)
+        {
+            if(done)
+                break;
+            HIP(hipMemsetAsync(dc.p, 0, 8, stream));
+            const uint64_t work = (pc.b1 - pc.b0) * B + (pc.o1 - pc.o0);
+            const uint64_t blocks = (work + TILE - 1) / TILE;
+            timed("collect", [&]{
+                if(map)
+                    hipLaunchKernelGGL(k_collect<true>, dim3(grid_for(k_collect<true>, blocks)), dim3(TILE), 0, stream, view(), pc.b0, pc.b1, pc.o0, pc.o1, dk.as<uint64_t>(), dv.as<uint64_t>(), dc.as<uint64_t>());
+                else
+                    hipLaunchKernelGGL(k_collect<false>, dim3(grid_for(k_collect<false>, blocks)), dim3(TILE), 0, stream, view(), pc.b0, pc.b1, pc.o0, pc.o1, dk.as<uint64_t>(), static_cast<uint64_t*>(nullptr), dc.as<uint64_t>());
+            });
+            uint64_t n = 0;
+            HIP(hipMemcpyAsync(&n, dc.p, 8, hipMemcpyDeviceToHost, stream));
+            HIP(hipStreamSynchronize(stream));
+            if(n == 0)
+                continue;
+            hk.resize(n);
+            HIP(hipMemcpyAsync(hk.data(), dk.p, n * 8, hipMemcpyDeviceToHost, stream));
+            if(map)
+            {
+                hv.resize(n);
+                HIP(hipMemcpyAsync(hv.data(), dv.p, n * 8, hipMemcpyDeviceToHost, stream));
+            }
+            HIP(hipStreamSynchronize(stream));
+            deliver(n);
+        }
+    }
+
+    uint64_t reduce(uint32_t what, uint64_t mask) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        DevBuf out;
+        out.reserve(8);
+        HIP(hipMemsetAsync(out.p, 0, 8, stream));
+        const uint64_t blocks = (cap * B + ov_slots + TILE - 1) / TILE;
+        timed("reduce", [&]{
+            if(map)
+                hipLaunchKernelGGL(k_reduce<true>, dim3(grid_for(k_reduce<true>, blocks)), dim3(TILE), 0, stream, view(), what, mask, out.as<uint64_t>());
+            else
+                hipLaunchKernelGGL(k_reduce<false>, dim3(grid_for(k_reduce<false>, blocks)), dim3(TILE), 0, stream, view(), what, mask, out.as<uint64_t>());
+        });
+        uint64_t r = 0;
+        HIP(hipMemcpyAsync(&r, out.p, 8, hipMemcpyDeviceToHost, stream));
+        HIP(hipStreamSynchronize(stream));
+        return r;
+    }
+
+    // capi/Kmer_Table.hpp:270-279 with the placement of
+    // include/Streaming_Kmer_Hash_Table.hpp:2728-2752.
+    void stats(kache_stats& s) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        s = kache_stats{};
+        s.capacity = cap * B;
+        s.bytes = cap * bucket_bytes();
+        s.keys = size;
+        s.overflow_keys = ov_count;
+        if(size == 0)
+            return;
+        DevBuf out;
+        out.reserve(16);
+        HIP(hipMemsetAsync(out.p, 0, 16, stream));
+        const uint64_t blocks = (cap * B + TILE - 1) / TILE;
+        if(map)
+            hipLaunchKernelGGL(k_placement<true>, dim3(grid_for(k_placement<true>, blocks)), dim3(TILE), 0, stream, view(), out.as<uint64_t>());
+        else
+            hipLaunchKernelGGL(k_placement<false>, dim3(grid_for(k_placement<false>, blocks)), dim3(TILE), 0, stream, view(), out.as<uint64_t>());
+        HIP(hipGetLastError());
+        uint64_t ps[2] = {0, 0};
+        HIP(hipMemcpyAsync(ps, out.p, 16, hipMemcpyDeviceToHost, stream));
+        HIP(hipStreamSynchronize(stream));
+        const double all = static_cast<double>(ps[0] + ps[1] + ov_count);
+        s.primary_pct = ps[0] * 100.0 / all;
+        s.secondary_pct = ps[1] * 100.0 / all;
+        s.overflow_pct = ov_count * 100.0 / all;
+    }
+};
+
+
+// ---------------------------------------------------------------------------
+// The C ABI.
+
+namespace
+{
+
+// capi/kache_capi.cpp:34-46.
+bool valid(const char* const* seqs, const size_t* lens, size_t n)
+{
+    if(n == 0)
+        return true;
+    if(seqs == nullptr || lens == nullptr)
+        return false;
+    for(size_t i = 0; i < n; ++i)
+        if(seqs[i] == nullptr && lens[i] > 0)
+            return false;
+    return true;
+}
+
+// capi/kache_capi.cpp:50-53.
+bool fits(const kache_table* t, uint64_t x)
+{
+    return t->value_bytes > 0 && (t->value_bytes == 8 || x >> (8 * t->value_bytes) == 0);
+}
+
+// capi/kache_capi.cpp:57-60.
+bool registered(const kache_table* t, uint32_t user) { return user < t->user_c.load(std::memory_order_acquire); }
+
+bool device_text_ok(const char* text, uint64_t bytes)
+{
+    return bytes == 0 || (text != nullptr && reinterpret_cast<uintptr_t>(text) % 16 == 0);
+}
+
+}
+
+extern "C"
+{
+
+int kache_supports(uint32_t k, uint32_t value_bytes) noexcept
+{
+    return k == 31 && (value_bytes == 0 || value_bytes == 1 || value_bytes == 2 || value_bytes == 4 || value_bytes == 8);
+}
+
+kache_options kache_default_options(uint32_t k, uint32_t value_bytes) noexcept
+{
+    return kache_options{k, value_bytes, uint64_t(1) << 22, 0.8, 16, 0};
+}
+
+int kache_new(const kache_options* options, kache_table** out) noexcept
+{
+    if(out == nullptr)
+        return KACHE_EINVAL;
+    *out = nullptr;
+    if(options == nullptr)
+        return KACHE_EINVAL;
+    const auto& o = *options;
+    if(!kache_supports(o.k, o.value_bytes) || !(o.load_factor > 0 && o.load_factor < 1) || o.resize_workers < 1 ||
+       o.pow2_capacity > 1 || !size_fits(o.expected_keys, o.load_factor, o.pow2_capacity != 0))
+        return KACHE_EINVAL;
+
+    return guard([&]()
+    {
+        int n = 0;
+        if(hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        {
+            (void)hipGetLastError();
+            last_error = "no HIP device";
+            return static_cast<int>(KACHE_EINTERNAL);
+        }
+        *out = new kache_table(o);
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+void kache_free(kache_table* t) noexcept
+{
+    delete t;
+}
+
+int kache_register(kache_table* t, uint32_t* user) noexcept
+{
+    if(t == nullptr || user == nullptr)
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const std::lock_guard<std::mutex> lock(t->reg);
+        const auto u = t->user_c.load(std::memory_order_relaxed);
+        if(u == KACHE_MAX_USERS)
+            return static_cast<int>(KACHE_EUSERS);
+        t->user_c.store(u + 1, std::memory_order_release);
+        *user = u;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_insert(kache_table* t, uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* inserted) noexcept
+{
+    if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) || (t->value_bytes > 0 && (op > KACHE_MAX || !fits(t, x))))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = t->insert_host(user, seqs, lens, n, t->map ? op : 0, false, t->map ? x : 0, 0);
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_insert_oriented(kache_table* t, uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x_fwd, uint64_t x_rev, uint64_t* inserted) noexcept
+{
+    const bool op_ok = (op == KACHE_ASSIGN || op == KACHE_ADD || op == KACHE_OR || op == KACHE_MAX);
+    if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) || !op_ok || !fits(t, x_fwd) || !fits(t, x_rev))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = t->insert_host(user, seqs, lens, n, op, true, x_fwd, x_rev);
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_update(kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* present) noexcept
+{
+    if(t == nullptr || !valid(seqs, lens, n) || op > KACHE_MAX || !fits(t, x))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = t->update_host(seqs, lens, n, op, x);
+        if(present != nullptr)
+            *present = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_count(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint64_t* counts) noexcept
+{
+    if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && counts == nullptr))
+        return KACHE_EINVAL;
+    return guard([&](){ t->per_sequence<Q_COUNT>(seqs, lens, n, counts, 0, 0); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_count_masked(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint64_t mask_fwd, uint64_t mask_rev, uint64_t* counts) noexcept
+{
+    if(t == nullptr || t->value_bytes == 0 || !valid(seqs, lens, n) || (n > 0 && counts == nullptr))
+        return KACHE_EINVAL;
+    // Masks act on values of the table's width (capi/Kmer_Table.hpp:146).
+    const uint64_t m = t->vmax();
+    return guard([&](){ t->per_sequence<Q_MASKED>(seqs, lens, n, counts, mask_fwd & m, mask_rev & m); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_sum(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint64_t* sums) noexcept
+{
+    if(t == nullptr || t->value_bytes == 0 || !valid(seqs, lens, n) || (n > 0 && sums == nullptr))
+        return KACHE_EINVAL;
+    return guard([&](){ t->per_sequence<Q_SUM>(seqs, lens, n, sums, 0, 0); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_lookup(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint8_t* found, void* values, uint8_t* fwd) noexcept
+{
+    if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && found == nullptr) || (t->value_bytes == 0 && values != nullptr))
+        return KACHE_EINVAL;
+    return guard([&](){ t->lookup(seqs, lens, n, found, values, fwd); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_get(const kache_table* t, const char* kmer, uint8_t* found, uint64_t* value) noexcept
+{
+    if(t == nullptr || kmer == nullptr || found == nullptr)
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        // The k characters at `kmer`, looked up as a one-position sequence
+        // (include/Streaming_Kmer_Hash_Table.hpp:2607-2645).
+        const size_t len = K;
+        uint8_t f = KACHE_NOT_KMER;
+        uint64_t v = 0;
+        t->lookup(&kmer, &len, 1, &f, t->map ? &v : nullptr, nullptr);
+        *found = (f == KACHE_PRESENT);
+        if(value != nullptr)
+            *value = (f == KACHE_PRESENT) ? v : 0;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_size(const kache_table* t, uint64_t* keys) noexcept
+{
+    if(t == nullptr || keys == nullptr)
+        return KACHE_EINVAL;
+    return guard([&](){ std::lock_guard<std::mutex> lock(t->mu); *keys = t->size; return static_cast<int>(KACHE_OK); });
+}
+
+int kache_reduce(const kache_table* t, uint32_t what, uint64_t mask, uint32_t workers, uint64_t* out) noexcept
+{
+    if(t == nullptr || out == nullptr || workers < 1 || what > KACHE_REDUCE_SUM || (t->value_bytes == 0 && what != KACHE_REDUCE_KEYS))
+        return KACHE_EINVAL;
+    return guard([&](){ *out = t->reduce(what, mask & t->vmax()); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_for_each(const kache_table* t, uint32_t shards, uint32_t shard, kache_keys_fn f, void* ctx) noexcept
+{
+    if(t == nullptr || f == nullptr || shard >= shards)
+        return KACHE_EINVAL;
+    return guard([&](){ t->for_each(shards, shard, f, ctx); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_get_stats(const kache_table* t, kache_stats* out) noexcept
+{
+    if(t == nullptr || out == nullptr)
+        return KACHE_EINVAL;
+    return guard([&](){ t->stats(*out); return static_cast<int>(KACHE_OK); });
+}
+
+const char* kache_strerror(int code) noexcept
+{
+    switch(code)
+    {
+    case KACHE_OK:          return "success";
+    case KACHE_EINVAL:      return "invalid argument";
+    case KACHE_EUSERS:      return "too many users registered";
+    case KACHE_ENOMEM:      return "out of memory";
+    case KACHE_EINTERNAL:   return "internal error";
+    default:                return "unknown status";
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Extensions (include/kache_amd.h).
+
+int kache_amd_insert_device(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) noexcept
+{
+    if(t == nullptr || !registered(t, user) || !device_text_ok(text, text_bytes) ||
+       (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = text_bytes ? t->insert_device(user, reinterpret_cast<const uint8_t*>(text), text_bytes, t->map ? op : 0, false, t->map ? x : 0, 0) : 0;
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x_fwd, uint64_t x_rev, uint64_t* inserted) noexcept
+{
+    const bool op_ok = (op == KACHE_ASSIGN || op == KACHE_ADD || op == KACHE_OR || op == KACHE_MAX);
+    if(t == nullptr || !registered(t, user) || !device_text_ok(text, text_bytes) || !op_ok || !fits(t, x_fwd) || !fits(t, x_rev))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = text_bytes ? t->insert_device(user, reinterpret_cast<const uint8_t*>(text), text_bytes, op, true, x_fwd, x_rev) : 0;
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) noexcept
+{
+    if(t == nullptr || !device_text_ok(text, text_bytes) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        uint64_t c = 0;
+        if(text_bytes)
+        {
+            std::lock_guard<std::mutex> lock(t->mu);
+            HIP(hipDeviceSynchronize());
+            c = t->update_text(op, reinterpret_cast<const uint8_t*>(text), text_bytes, x);
+        }
+        if(present != nullptr)
+            *present = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text_bytes, uint64_t* hits, uint64_t* positions) noexcept
+{
+    if(t == nullptr || hits == nullptr || !device_text_ok(text, text_bytes))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        Counters c{};
+        if(text_bytes)
+            c = t->count_device(reinterpret_cast<const uint8_t*>(text), text_bytes);
+        *hits = c.present;
+        if(positions != nullptr)
+            *positions = c.positions;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_describe_device(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) noexcept
+{
+    if(!device_text_ok(text, text_bytes) || (text_bytes > 0 && (!key || !h || !c || !fwd || !valid)))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        if(text_bytes == 0)
+            return static_cast<int>(KACHE_OK);
+        const uint64_t tiles = (text_bytes + TILE - 1) / TILE;
+        const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>(tiles, 4096));
+        hipLaunchKernelGGL(k_describe, dim3(g), dim3(TILE), 0, 0, reinterpret_cast<const uint8_t*>(text), text_bytes, 0, text_bytes, key, h, c, fwd, valid);
+        HIP(hipGetLastError());
+        HIP(hipDeviceSynchronize());
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_profile(kache_table* t, int on) noexcept
+{
+    if(t == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    t->profiling = (on != 0);
+    if(on)
+        t->prof.clear();
+    return KACHE_OK;
+}
+
+int kache_amd_kernel_time(const kache_table* t, const char* name, double* ms, uint64_t* launches) noexcept
+{
+    if(t == nullptr || name == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    const auto it = t->prof.find(name);
+    if(ms)
+        *ms = it == t->prof.end() ? 0 : it->second.first;
+    if(launches)
+        *launches = it == t->prof.end() ? 0 : it->second.second;
+    return KACHE_OK;
+}
+
+int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) noexcept
+{
+    if(t == nullptr || blocks > 64)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    t->blocks_per_cu = blocks;
+    return KACHE_OK;
+}
+
+int kache_amd_get_info(const kache_table* t, kache_amd_info* out) noexcept
+{
+    if(t == nullptr || out == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    out->buckets = t->cap;
+    out->bucket_bytes = t->bucket_bytes();
+    out->overflow_slots = t->ov_slots;
+    out->overflow_keys = t->ov_count;
+    out->resizes = t->resizes;
+    out->device_bytes = t->cap * t->bucket_bytes() + t->ov_slots * t->slot_words() * 8;
+    return KACHE_OK;
+}
+
+const char* kache_amd_last_error(void) noexcept
+{
+    return last_error.c_str();
+}
+
+uint64_t kache_amd_host_minimizer_hash(uint64_t key) noexcept { return minimizer_hash(key); }
+uint8_t kache_amd_host_checksum(uint64_t key) noexcept { return checksum(key); }
+uint64_t kache_amd_host_xxh3(uint64_t x, uint64_t seed) noexcept { return xxh3_8(x, seed); }
+uint64_t kache_amd_host_cap_for(uint64_t n, double lf, uint32_t pow2) noexcept { return cap_for(n, lf, pow2 != 0); }
+
+int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) noexcept
+{
+    if(text_bytes > 0 && (!text || !key || !h || !c || !fwd || !valid))
+        return KACHE_EINVAL;
+    // The device tile's steps, on the host: 32-character words packed with
+    // pack32, 32-base windows from two words, l-mer hashes, window minima.
+    const uint64_t words = (text_bytes + TILE_WORDS * 32) / 32 + 2;
+    std::vector<uint64_t> pk(words, 0);
+    std::vector<uint32_t> bad(words, ~0u);
+    for(uint64_t w = 0; w < words; ++w)
+    {
+        uint32_t d[8];
+        for(int i = 0; i < 8; ++i)
+        {
+            uint32_t v = 0;
+            for(int b = 0; b < 4; ++b)
+            {
+                const uint64_t at = w * 32 + 4 * i + b;
+                const uint32_t ch = at < text_bytes ? static_cast<uint8_t>(text[at]) : '\n';
+                v |= ch << (8 * b);
+            }
+            d[i] = v;
+        }
+        pack32(d, pk[w], bad[w]);
+    }
+    const auto window = [&](uint64_t u)
+    {
+        const uint64_t wi = u >> 5;
+        const int sh = static_cast<int>(u & 31) * 2;
+        return sh ? (pk[wi] << sh) | (pk[wi + 1] >> (64 - sh)) : pk[wi];
+    };
+    std::vector<uint64_t> lh(text_bytes + W);
+    for(uint64_t p = 0; p < text_bytes + W - 1; ++p)
+    {
+        uint64_t kf, kr;
+        window_parts(window(p), kf, kr, lh[p]);
+    }
+    for(uint64_t p = 0; p < text_bytes; ++p)
+    {
+        uint64_t kf, kr, l0;
+        window_parts(window(p), kf, kr, l0);
+        const uint64_t wi = p >> 5;
+        const uint64_t b64 = static_cast<uint64_t>(bad[wi]) | (static_cast<uint64_t>(bad[wi + 1]) << 32);
+        const bool ok = ((b64 >> (p & 31)) & 0x7FFFFFFFull) == 0 && p + K <= text_bytes;
+        uint64_t m = lh[p];
+        for(int o = 1; o < W; ++o)
+            m = std::min(m, lh[p + o]);
+        valid[p] = ok;
+        key[p] = ok ? std::min(kf, kr) : 0;
+        h[p] = ok ? m : 0;
+        c[p] = ok ? checksum(std::min(kf, kr)) : 0;
+        fwd[p] = ok ? (kf <= kr) : 0;
+    }
+    return KACHE_OK;
+}
+
+}
