@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--genome", type=int, default=100_000_000)
     p.add_argument("--expected", type=int, default=100_000_000)
     p.add_argument("--lookup-reads", type=int, default=2_000_000)
-    p.add_argument("--cpu-sample-reads", type=int, default=2_000_000)
+    p.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--blocks-per-cu", type=int, default=0)

@@ -16,7 +16,8 @@ _api = None
 
 
 def library_path():
-    return capi.library_path()
+    """The built library; KACHE_AMD_LIB names another build of it (A/B runs)."""
+    return os.environ.get("KACHE_AMD_LIB") or capi.library_path()
 
 
 def api():

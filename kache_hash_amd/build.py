@@ -36,14 +36,24 @@ def command(out=LIB, extra=()):
     ]
 
 
+ROOF = os.path.join(HERE, "memory_roof")
+
+
 def build(verbose=True):
-    """Compiles the library (always) and returns its path."""
+    """Compiles the library and the memory-roof probe (always); returns the
+    library's path."""
     tmp = LIB + ".tmp"
     cmd = command(tmp)
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=ROOT)
     os.replace(tmp, LIB)
+    roof = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops", os.path.join(HERE, "csrc", "memory_roof.hip"), "-o", ROOF + ".tmp"]
+    if verbose:
+        print(" ".join(roof), flush=True)
+    subprocess.run(roof, check=True, cwd=ROOT)
+    os.replace(ROOF + ".tmp", ROOF)
     return LIB
 
 

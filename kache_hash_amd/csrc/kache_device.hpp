@@ -479,22 +479,43 @@ __device__ __forceinline__ uint64_t window_at(const TileLds& s, int u)
 }
 
 // `text + p0` must be 16-byte aligned with TILE_WORDS * 32 readable bytes.
-// Positions at or past `pos_end` are not described; a k-mer must also end
-// within the first `text_len` bytes.
-__device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, uint64_t pos_end, uint64_t text_len, TileLds& s)
+// The 32-byte text words a tile's first TILE_WORDS threads stage, loaded a
+// tile ahead so their latency hides behind the previous tile's probes.
+struct TileWords
+{
+    uint4 a, b;
+};
+
+// Starts loading the words of the tile at p0 (threads below TILE_WORDS).
+// `text + p0` must be 16-byte aligned with TILE_WORDS * 32 readable bytes.
+__device__ __forceinline__ void load_tile(const uint8_t* text, uint64_t p0, TileWords& w)
+{
+    if(threadIdx.x < TILE_WORDS)
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(text + p0 + 32 * threadIdx.x);
+        w.a = src[0], w.b = src[1];
+    }
+}
+
+// Describes the tile at p0 from its loaded words `w`, then starts loading
+// the words of the tile at `p_next` into `w` (if `has_next`).  Positions at
+// or past `pos_end` are not described; a k-mer must also end within the
+// first `text_len` bytes.
+__device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, uint64_t pos_end, uint64_t text_len, TileLds& s,
+                                              TileWords& w, uint64_t p_next, bool has_next)
 {
     const int t = threadIdx.x;
     if(t < TILE_WORDS)
     {
-        const uint4* src = reinterpret_cast<const uint4*>(text + p0 + 32 * t);
-        const uint4 a = src[0], b = src[1];
-        const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const uint32_t d[8] = {w.a.x, w.a.y, w.a.z, w.a.w, w.b.x, w.b.y, w.b.z, w.b.w};
         uint64_t pk;
         uint32_t bad;
         pack32(d, pk, bad);
         s.pk[t] = pk, s.bad[t] = bad;
     }
     __syncthreads();
+    if(has_next)
+        load_tile(text, p_next, w);
 
     Desc d;
     uint64_t kf, kr, lh;
@@ -524,6 +545,23 @@ __device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, 
     d.fwd = kf <= kr;
     d.c = checksum(d.key);
     return d;
+}
+
+// Walks the tiles [0, tiles) of a range from p_begin, block by block, each
+// tile's text prefetched during the previous one: calls f(tile, desc).
+template <typename F>
+__device__ __forceinline__ void for_tiles(const uint8_t* text, uint64_t p_begin, uint64_t pos_end, uint64_t text_len, TileLds& s, const F& f)
+{
+    const uint64_t tiles = (pos_end - p_begin + TILE - 1) / TILE;
+    TileWords w{};
+    if(blockIdx.x < tiles)
+        load_tile(text, p_begin + blockIdx.x * uint64_t(TILE), w);
+    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        const uint64_t next = tile + gridDim.x;
+        const Desc d = describe_tile(text, p_begin + tile * TILE, pos_end, text_len, s, w, p_begin + next * TILE, next < tiles);
+        f(tile, d);
+    }
 }
 
 // Sums `v` over the block into `*dst` with one atomic.

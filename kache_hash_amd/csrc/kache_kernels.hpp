@@ -5,6 +5,17 @@
 
 #include "kache_device.hpp"
 
+// Occupancy of the streaming kernels (insert, update, query): minimum waves
+// per SIMD the compiler must fit, which caps their registers.
+#ifndef KAMD_STREAM_WAVES
+#define KAMD_STREAM_WAVES 8
+#endif
+#if KAMD_STREAM_WAVES > 0
+#define KAMD_STREAM_BOUNDS __launch_bounds__(256, KAMD_STREAM_WAVES)
+#else
+#define KAMD_STREAM_BOUNDS __launch_bounds__(256)
+#endif
+
 namespace kamd
 {
 
@@ -21,30 +32,27 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 // valid position of [pos_begin, pos_end) of `text`.  `x` is the operand, or
 // the forward one with ORIENT (x_rev for k-mers occurring reverse-complemented).
 template <bool MAP, uint32_t OP, bool ORIENT>
-__global__ __launch_bounds__(TILE) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
+__global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
 {
     __shared__ TileLds s;
     __shared__ uint64_t red[TILE / 64];
     uint64_t ins = 0, ins_ov = 0, failed = 0;
-    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
-    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
-        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
-        if(d.valid)
+        if(!d.valid)
+            return;
+        uint64_t* slot;
+        bool in_ov;
+        const Res r = insert_key<MAP>(T, d.key, d.h, d.c, slot, in_ov);
+        if(r == RES_FULL)
+            failed++;
+        else
         {
-            uint64_t* slot;
-            bool in_ov;
-            const Res r = insert_key<MAP>(T, d.key, d.h, d.c, slot, in_ov);
-            if(r == RES_FULL)
-                failed++;
-            else
-            {
-                ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
-                if constexpr(MAP)
-                    value_op<OP>(slot + 1, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
-            }
+            ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
+            if constexpr(MAP)
+                value_op<OP>(slot + 1, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
         }
-    }
+    });
     block_add(ins, &cnt->inserted, red);
     block_add(ins_ov, &cnt->ov_inserted, red);
     block_add(failed, &cnt->failed, red);
@@ -52,22 +60,20 @@ __global__ __launch_bounds__(TILE) void k_insert(TableView T, const uint8_t* __r
 
 // Present-only updates (include/Streaming_Kmer_Hash_Table.hpp:2648-2691).
 template <uint32_t OP>
-__global__ __launch_bounds__(TILE) void k_update(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, Counters* cnt)
+__global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, Counters* cnt)
 {
     __shared__ TileLds s;
     __shared__ uint64_t red[TILE / 64];
     uint64_t present = 0;
-    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
-    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
-        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
         if(d.valid)
             if(uint64_t* slot = find_key<true>(T, d.key, d.h, d.c))
             {
                 present++;
                 value_op<OP>(slot + 1, x, T.vmax, false);
             }
-    }
+    });
     block_add(present, &cnt->present, red);
 }
 
@@ -109,16 +115,14 @@ __device__ __forceinline__ uint32_t segment_of(const QueryArgs& a, uint64_t tile
 }
 
 template <bool MAP, int MODE>
-__global__ __launch_bounds__(TILE) void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_end, QueryArgs a, Counters* cnt)
+__global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_end, QueryArgs a, Counters* cnt)
 {
     __shared__ TileLds s;
     __shared__ uint64_t red[TILE / 64];
     uint64_t total = 0, positions = 0;
-    const uint64_t tiles = (pos_end + TILE - 1) / TILE;
-    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    for_tiles(text, 0, pos_end, pos_end, s, [&](uint64_t tile, const Desc& d)
     {
         const uint64_t p = tile * TILE + threadIdx.x;
-        const Desc d = describe_tile(text, tile * TILE, pos_end, pos_end, s);
         uint64_t v = 0;
         bool hit = false;
         if(d.valid)
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(TILE) void k_query(TableView T, const uint8_t* __re
                 pending &= ~__ballot(mine);
             }
         }
-    }
+    });
     if constexpr(MODE == Q_TOTAL)
     {
         block_add(total, &cnt->present, red);
@@ -183,8 +187,11 @@ __global__ __launch_bounds__(TILE) void k_query(TableView T, const uint8_t* __re
 // (include/Streaming_Kmer_Hash_Table.hpp:1473-1681), out of place.
 // Phase 1: a key at its primary bucket b moves to 2b or 2b + 1, in its slot
 // order; half a wave per old bucket, so the order is a ballot prefix count.
+// The keys away from their primary bucket are marked in `away`: bit j of
+// word w is slot j % 32 of old bucket 2w + j / 32, written by one lane per
+// wave, with no atomics.
 template <bool MAP>
-__global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N)
+__global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64_t* away)
 {
     const uint32_t lane = threadIdx.x & 63, j = lane & 31;
     const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N)
     for(uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; w < pairs; w += (uint64_t(gridDim.x) * blockDim.x) >> 6)
     {
         const uint64_t b = 2 * w + (lane >> 5);
-        bool lo = false, hi = false;
+        bool lo = false, hi = false, aw = false;
         uint64_t sk = 0, val = 0;
         uint8_t c = 0;
         if(b < O.cap)
@@ -212,72 +219,103 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N)
                     hi = (bucket_of(h, N.cap) == 2 * b + 1);
                     lo = !hi;
                 }
+                else
+                    aw = true;
             }
         }
-        const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask;
-        if(lo || hi)
+        const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask, m_away = __ballot(aw);
+        if(b < O.cap)
         {
-            const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
-            uint8_t* nb = bucket_at<MAP>(N, 2 * b + (hi ? 1 : 0));
-            uint64_t* ns = slot_at<MAP>(nb, r);
-            ns[0] = sk;
-            if(MAP)
-                ns[1] = val;
-            nb[r] = c;
+            // Lane j of the half writes slot j of both new buckets: the key
+            // ranked j among those moving there, or an empty slot; the new
+            // buckets need no zeroing beforehand.
+            const uint32_t n_lo = __popcll(m_lo), n_hi = __popcll(m_hi);
+            uint8_t* nlo = bucket_at<MAP>(N, 2 * b);
+            uint8_t* nhi = nlo + Geo<MAP>::bucket_bytes;
+            if(lo || hi)
+            {
+                const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
+                uint8_t* nb = lo ? nlo : nhi;
+                uint64_t* ns = slot_at<MAP>(nb, r);
+                ns[0] = sk;
+                if(MAP)
+                    ns[1] = val;
+                nb[r] = c;
+            }
+            if(j >= n_lo)
+            {
+                uint64_t* ns = slot_at<MAP>(nlo, j);
+                ns[0] = 0;
+                if(MAP)
+                    ns[1] = 0;
+                nlo[j] = 0;
+            }
+            if(j >= n_hi)
+            {
+                uint64_t* ns = slot_at<MAP>(nhi, j);
+                ns[0] = 0;
+                if(MAP)
+                    ns[1] = 0;
+                nhi[j] = 0;
+            }
+            if(j < 8)   // the spill filters start clear
+            {
+                reinterpret_cast<uint32_t*>(nlo + 32)[j] = 0;
+                reinterpret_cast<uint32_t*>(nhi + 32)[j] = 0;
+            }
         }
+        if(lane == 0)
+            away[w] = m_away;
     }
 }
 
-// Phase 2: the keys away from their primary buckets, and the old overflow
-// table's, go in with the insertion policy, checksums and minimizers
-// recomputed (:1575-1665).
+// Re-inserts a key with the insertion policy, its checksum and minimizer
+// recomputed (:1305-1371).
 template <bool MAP>
-__global__ __launch_bounds__(TILE) void k_reinsert(TableView O, TableView N, Counters* cnt)
+__device__ __forceinline__ void reinsert_one(const TableView& N, uint64_t sk, uint64_t val, uint64_t& ov_n, uint64_t& failed)
+{
+    uint64_t* slot;
+    bool in_ov;
+    const uint64_t key = ~sk;
+    const Res r = insert_key<MAP>(N, key, minimizer_hash(key), checksum(key), slot, in_ov);
+    if(r == RES_FULL)
+        failed++;
+    else
+    {
+        ov_n += in_ov;
+        if(MAP)
+            st_rel(slot + 1, val);
+    }
+}
+
+// Phase 2: the keys marked away from their primary buckets, then the old
+// overflow table's, go in with the insertion policy (:1575-1665).  A lane
+// takes one word of marks (two old buckets) and walks its set bits, so every
+// lane has work; then one overflow slot.
+template <bool MAP>
+__global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableView O, TableView N, Counters* cnt)
 {
     __shared__ uint64_t red[TILE / 64];
     uint64_t ov_n = 0, failed = 0;
-    const uint64_t main_slots = O.cap * B, all = main_slots + O.ov_mask + 1;
+    const uint64_t pairs = (O.cap + 1) / 2, all = pairs + O.ov_mask + 1;
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
     {
-        uint64_t sk, val = 0, h;
-        if(g < main_slots)
+        if(g < pairs)
         {
-            const uint64_t b = g / B;
-            const uint32_t j = g % B;
-            uint8_t* ob = bucket_at<MAP>(O, b);
-            if(!ob[j])
-                continue;
-            const uint64_t* os = slot_at<MAP>(ob, j);
-            sk = os[0];
-            if(MAP)
-                val = os[1];
-            h = minimizer_hash(~sk);
-            if(bucket_of(h, O.cap) == b)
-                continue;
+            for(uint64_t m = away[g]; m; m &= m - 1)
+            {
+                const uint32_t bit = __builtin_ctzll(m);
+                const uint64_t* os = slot_at<MAP>(bucket_at<MAP>(O, 2 * g + (bit >> 5)), bit & 31);
+                reinsert_one<MAP>(N, os[0], MAP ? os[1] : 0, ov_n, failed);
+            }
         }
         else
         {
-            const uint64_t* os = O.ov + (g - main_slots) * Geo<MAP>::slot_words;
-            sk = os[0];
-            if(!sk)
-                continue;
-            if(MAP)
-                val = os[1];
-            h = minimizer_hash(~sk);
-        }
-
-        uint64_t* slot;
-        bool in_ov;
-        const uint64_t key = ~sk;
-        const Res r = insert_key<MAP>(N, key, h, checksum(key), slot, in_ov);
-        if(r == RES_FULL)
-            failed++;
-        else
-        {
-            ov_n += in_ov;
-            if(MAP)
-                st_rel(slot + 1, val);
+            const uint64_t* os = O.ov + (g - pairs) * Geo<MAP>::slot_words;
+            const uint64_t sk = os[0];
+            if(sk)
+                reinsert_one<MAP>(N, sk, MAP ? os[1] : 0, ov_n, failed);
         }
     }
     block_add(ov_n, &cnt->ov_inserted, red);
@@ -437,11 +475,9 @@ __global__ void k_zero64(uint64_t* p, uint64_t n)
 __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid)
 {
     __shared__ TileLds s;
-    const uint64_t tiles = (pos_end - pos_begin + TILE - 1) / TILE;
-    for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t tile, const Desc& d)
     {
         const uint64_t p = pos_begin + tile * TILE + threadIdx.x;
-        const Desc d = describe_tile(text, pos_begin + tile * TILE, pos_end, text_len, s);
         if(p < pos_end)
         {
             const uint64_t i = p - pos_begin;
@@ -451,7 +487,7 @@ __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ t
             c[i] = d.valid ? d.c : 0;
             fwd[i] = d.valid ? static_cast<uint8_t>(d.fwd) : 0;
         }
-    }
+    });
 }
 
 }

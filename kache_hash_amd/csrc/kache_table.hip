@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -231,13 +232,18 @@ struct kache_table
     // Profiling.
     bool profiling = false;
     mutable std::map<std::string, std::pair<double, uint64_t>> prof;
+    mutable std::vector<hipEvent_t> ev_pool;
+    mutable std::vector<std::pair<const char*, size_t>> ev_pending;
+    mutable size_t ev_used = 0;
 
     kache_table(const kache_options& o)
         : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0)
     {
         HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
         d_cnt = dev_alloc<Counters>(1);
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(Counters), hipHostMallocDefault));
+        zero_counters();
         try
         {
             cap = cap_for(o.expected_keys, lf, o.pow2_capacity != 0);
@@ -269,6 +275,9 @@ struct kache_table
             (void)hipFree(d_cnt), d_cnt = nullptr;
         if(h_cnt)
             (void)hipHostFree(h_cnt), h_cnt = nullptr;
+        for(auto e : ev_pool)
+            (void)hipEventDestroy(e);
+        ev_pool.clear();
         if(stream)
             (void)hipStreamDestroy(stream), stream = nullptr;
     }
@@ -316,7 +325,8 @@ struct kache_table
         return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, work_blocks)));
     }
 
-    // Launches `launch()` and, when profiling, brackets it with events.
+    // Launches `launch()` and, when profiling, brackets it with events from a
+    // pool; their times are summed at the next synchronization of the stream.
     template <typename F>
     void timed(const char* name, const F& launch) const
     {
@@ -326,19 +336,36 @@ struct kache_table
             HIP(hipGetLastError());
             return;
         }
-        hipEvent_t a, b;
-        HIP(hipEventCreate(&a));
-        HIP(hipEventCreate(&b));
+        if(ev_used + 2 > ev_pool.size())
+            for(int i = 0; i < 2; ++i)
+            {
+                hipEvent_t e;
+                HIP(hipEventCreate(&e));
+                ev_pool.push_back(e);
+            }
+        hipEvent_t a = ev_pool[ev_used], b = ev_pool[ev_used + 1];
         HIP(hipEventRecord(a, stream));
         launch();
         HIP(hipGetLastError());
         HIP(hipEventRecord(b, stream));
-        HIP(hipEventSynchronize(b));
-        float ms = 0;
-        HIP(hipEventElapsedTime(&ms, a, b));
-        (void)hipEventDestroy(a), (void)hipEventDestroy(b);
-        auto& e = prof[name];
-        e.first += ms, e.second++;
+        ev_pending.push_back({name, ev_used});
+        ev_used += 2;
+    }
+
+    // Synchronizes the stream, then sums the times of the launches profiled
+    // since the last synchronization.
+    void sync() const
+    {
+        HIP(hipStreamSynchronize(stream));
+        for(const auto& [name, i] : ev_pending)
+        {
+            float ms = 0;
+            HIP(hipEventElapsedTime(&ms, ev_pool[i], ev_pool[i + 1]));
+            auto& e = prof[name];
+            e.first += ms, e.second++;
+        }
+        ev_pending.clear();
+        ev_used = 0;
     }
 
     void zero_counters() const
@@ -347,11 +374,15 @@ struct kache_table
         HIP(hipGetLastError());
     }
 
+    // Reads the counters of the launches since the last read, and zeroes them
+    // again for the next (they are zero whenever no launch is in flight).
     Counters read_counters() const
     {
         HIP(hipMemcpyAsync(h_cnt, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream));
-        HIP(hipStreamSynchronize(stream));
-        return *h_cnt;
+        sync();
+        const Counters c = *h_cnt;
+        zero_counters();
+        return c;
     }
 
     // --- Overflow growth (include/Streaming_Kmer_Hash_Table.hpp:1412-1443) ---
@@ -360,7 +391,6 @@ struct kache_table
     {
         uint64_t* nov = alloc_ov(new_slots);
         const TableView N = view_of(buckets, cap, nov, new_slots);
-        zero_counters();
         const uint64_t blocks = (ov_slots + TILE - 1) / TILE;
         timed("ov_rehash", [&]{
             if(map)
@@ -399,7 +429,7 @@ struct kache_table
         const uint64_t ncap = 2 * cap;
         // :1387-1392: the new overflow table takes the old one's keys' room, at least.
         const uint64_t nov_slots = std::max<uint64_t>(ov_slots_for(std::max<uint64_t>(OV_MIN, ncap)), ov_slots);
-        uint8_t* nb = alloc_buckets(ncap);
+        uint8_t* nb = dev_alloc<uint8_t>(ncap * bucket_bytes());   // written whole by the split
         uint64_t* nov = nullptr;
         try
         {
@@ -412,20 +442,23 @@ struct kache_table
         }
 
         const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
-        zero_counters();
-        const uint64_t split_blocks = (cap * 32 + TILE - 1) / TILE;
-        const uint64_t re_blocks = (cap * B + ov_slots + TILE - 1) / TILE;
+        // One bit per old slot marks the keys away from their primary bucket.
+        const uint64_t pairs = (cap + 1) / 2;
+        DevBuf away;
+        away.reserve(pairs * 8);
+        const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
+        const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
         timed("split", [&]{
             if(map)
-                hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N);
+                hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
             else
-                hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N);
+                hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
         });
         timed("reinsert", [&]{
             if(map)
-                hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, O, N, d_cnt);
+                hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
             else
-                hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, O, N, d_cnt);
+                hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
         });
         const Counters c = read_counters();
         if(c.failed)
@@ -601,7 +634,6 @@ struct kache_table
         {
             const uint64_t e = std::min(text_len, b + LAUNCH_POS);
             ov_room(e - b);
-            zero_counters();
             timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
             const Counters c = read_counters();
             if(c.failed)
@@ -636,7 +668,6 @@ struct kache_table
     uint64_t insert_device(uint32_t user, const uint8_t* text, uint64_t text_len, uint32_t op, bool oriented, uint64_t x, uint64_t xr)
     {
         std::lock_guard<std::mutex> lock(mu);
-        HIP(hipDeviceSynchronize());
         CallSched s = sched_begin(user);
         uint64_t inserted = 0;
         try
@@ -663,7 +694,6 @@ struct kache_table
 
     uint64_t update_text(uint32_t op, const uint8_t* text, uint64_t text_len, uint64_t x) const
     {
-        zero_counters();
         timed("update", [&]{
             switch(op)
             {
@@ -759,12 +789,12 @@ struct kache_table
             a.per_seq = d_out.as<uint64_t>();
             a.mask_fwd = mf, a.mask_rev = mr;
             launch_query<MODE>(text, st.text_len, a);
-            HIP(hipStreamSynchronize(stream));  // the staging buffers are reused
+            sync();  // the staging buffers are reused
         }
         if(any)
         {
             HIP(hipMemcpyAsync(out, d_out.p, n * 8, hipMemcpyDeviceToHost, stream));
-            HIP(hipStreamSynchronize(stream));
+            sync();
         }
     }
 
@@ -806,14 +836,14 @@ struct kache_table
             a.fwd = fwd ? d_out3.as<uint8_t>() : nullptr;
             a.vbytes = vb;
             launch_query<Q_LOOKUP>(text, st.text_len, a);
-            HIP(hipStreamSynchronize(stream));
+            sync();
         }
         HIP(hipMemcpyAsync(found, d_out.p, p, hipMemcpyDeviceToHost, stream));
         if(values)
             HIP(hipMemcpyAsync(values, d_out2.p, p * vb, hipMemcpyDeviceToHost, stream));
         if(fwd)
             HIP(hipMemcpyAsync(fwd, d_out3.p, p, hipMemcpyDeviceToHost, stream));
-        HIP(hipStreamSynchronize(stream));
+        sync();
     }
 
     void fill(void* p, uint64_t n, uint8_t v) const
@@ -826,8 +856,6 @@ struct kache_table
     Counters count_device(const uint8_t* text, uint64_t text_len) const
     {
         std::lock_guard<std::mutex> lock(mu);
-        HIP(hipDeviceSynchronize());
-        zero_counters();
         QueryArgs a{};
         launch_query<Q_TOTAL>(text, text_len, a);
         return read_counters();
@@ -892,7 +920,7 @@ struct kache_table
             });
             uint64_t n = 0;
             HIP(hipMemcpyAsync(&n, dc.p, 8, hipMemcpyDeviceToHost, stream));
-            HIP(hipStreamSynchronize(stream));
+            sync();
             if(n == 0)
                 continue;
             hk.resize(n);
@@ -902,7 +930,7 @@ struct kache_table
                 hv.resize(n);
                 HIP(hipMemcpyAsync(hv.data(), dv.p, n * 8, hipMemcpyDeviceToHost, stream));
             }
-            HIP(hipStreamSynchronize(stream));
+            sync();
             deliver(n);
         }
     }
@@ -922,7 +950,7 @@ struct kache_table
         });
         uint64_t r = 0;
         HIP(hipMemcpyAsync(&r, out.p, 8, hipMemcpyDeviceToHost, stream));
-        HIP(hipStreamSynchronize(stream));
+        sync();
         return r;
     }
 
@@ -949,7 +977,7 @@ struct kache_table
         HIP(hipGetLastError());
         uint64_t ps[2] = {0, 0};
         HIP(hipMemcpyAsync(ps, out.p, 16, hipMemcpyDeviceToHost, stream));
-        HIP(hipStreamSynchronize(stream));
+        sync();
         const double all = static_cast<double>(ps[0] + ps[1] + ov_count);
         s.primary_pct = ps[0] * 100.0 / all;
         s.secondary_pct = ps[1] * 100.0 / all;
@@ -1224,7 +1252,6 @@ int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_byte
         if(text_bytes)
         {
             std::lock_guard<std::mutex> lock(t->mu);
-            HIP(hipDeviceSynchronize());
             c = t->update_text(op, reinterpret_cast<const uint8_t*>(text), text_bytes, x);
         }
         if(present != nullptr)
