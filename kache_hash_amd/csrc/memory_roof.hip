@@ -61,11 +61,15 @@ __global__ void k_stream(const uint4* a, uint64_t n16, uint64_t* sink)
         atomicAdd(reinterpret_cast<unsigned long long*>(sink), 1ull);
 }
 
-// MODE 0 read, 1 read-modify-write, 2 atomic add, 3 atomic add with return.
+// MODE 0 read, 1 read-modify-write, 2 atomic add, 3 atomic add with return,
+// 4 atomic add by groups of 4 lanes on consecutive 16-byte slots of one line
+// (the pattern of a minimizer run's counters).
 template <int MODE>
 __global__ void k_random(uint64_t* a, uint64_t lines, uint32_t line_words, uint32_t touch, uint64_t iters, uint64_t seed, uint64_t* sink)
 {
-    const uint64_t tid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    const uint64_t gtid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    const uint64_t tid = MODE == 4 ? gtid / 4 : gtid;
+    const uint32_t sub = MODE == 4 ? static_cast<uint32_t>(gtid % 4) * 2 : 0;
     uint64_t acc = 0;
     for(uint64_t it = 0; it < iters; it += DEPTH)
     {
@@ -97,11 +101,11 @@ __global__ void k_random(uint64_t* a, uint64_t lines, uint32_t line_words, uint3
             for(int d = 0; d < DEPTH; ++d)
                 p[d][1] = v[d] + 1;
         }
-        else if constexpr(MODE == 2)
+        else if constexpr(MODE == 2 || MODE == 4)
         {
 #pragma unroll
             for(int d = 0; d < DEPTH; ++d)
-                __hip_atomic_fetch_add(p[d] + 1, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(p[d] + 1 + sub, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         else
         {
@@ -196,6 +200,8 @@ int main(int argc, char** argv)
         phase("random_update", k_random<1>, 1);
         phase("random_atomic", k_random<2>, 2);
         phase("random_atomic_ret", k_random<3>, 3);
+        if(line == 64)
+            phase("run4_atomic", k_random<4>, 4);   // counts operations: 4 per line
         const uint64_t steps = 256;
         const double tc = timed([&]{ hipLaunchKernelGGL(k_chase, dim3(grid), dim3(block), 0, 0, a, lines, uint32_t(lw), steps, sink); });
         std::printf(", \"chase_%u_ns\": %.1f, \"chase_%u_lines_per_s\": %.4g", line, tc / steps * 1e9, line, lanes * double(steps) / tc);
