@@ -29,6 +29,17 @@ int kache_amd_insert_device(kache_table* t, uint32_t user, const char* text, uin
 int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x_fwd, uint64_t x_rev, uint64_t* inserted) KACHE_NOEXCEPT;
 int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
 
+/* Batches of one device text: batch i is [bounds[i], bounds[i + 1]), each
+ * ending with a non-ACGT byte.  kache_amd_insert_batches inserts them in
+ * order, each as one kache_amd_insert_device call of `user` (its count of
+ * new keys at inserted[i], unless `inserted` is NULL), with the launches
+ * pipelined: a batch starts before the previous one's count is read
+ * whenever no doubling can fall due in between.  kache_amd_count_batches
+ * looks them up back to back: present positions at hits[i] and, unless
+ * NULL, all k-mer positions at positions[i]. */
+int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, const uint64_t* bounds, uint64_t n_batches, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_count_batches(const kache_table* t, const char* text, const uint64_t* bounds, uint64_t n_batches, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;
+
 /* The number of k-mer positions of a device text present in the table
  * (`hits`) and, unless NULL, of all its k-mer positions (`positions`). */
 int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text_bytes, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;

@@ -69,6 +69,8 @@ AMD_SYMBOLS = {
     "kache_amd_insert_oriented_device": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_update_device": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_count_device": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "kache_amd_insert_batches": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint64, C.c_uint32, C.c_uint64, _P]),
+    "kache_amd_count_batches": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
     "kache_amd_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
     "kache_amd_device_free": (C.c_int, [_P]),
     "kache_amd_copy": (C.c_int, [_P, _P, C.c_uint64, C.c_int]),
@@ -302,6 +304,22 @@ class Table:
         n = C.c_uint64()
         self.api.check(self.api.lib.kache_amd_insert_oriented_device(self.h, user, text_ptr, text_bytes, op, x_fwd, x_rev, C.byref(n)), "kache_amd_insert_oriented_device")
         return n.value
+
+    def insert_batches(self, text_ptr, bounds, op=KEEP, x=0, user=0):
+        """Pipelined inserts of the batches [bounds[i], bounds[i+1]) of a device text; per-batch new-key counts."""
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        n = len(b) - 1
+        out = np.zeros(max(n, 1), np.uint64)
+        self.api.check(self.api.lib.kache_amd_insert_batches(self.h, user, text_ptr, b.ctypes.data, n, op, x, out.ctypes.data), "kache_amd_insert_batches")
+        return out[:n]
+
+    def count_batches(self, text_ptr, bounds):
+        """(hits, positions) per batch of a device text."""
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        n = len(b) - 1
+        h, p = np.zeros(max(n, 1), np.uint64), np.zeros(max(n, 1), np.uint64)
+        self.api.check(self.api.lib.kache_amd_count_batches(self.h, text_ptr, b.ctypes.data, n, h.ctypes.data, p.ctypes.data), "kache_amd_count_batches")
+        return h[:n], p[:n]
 
     def update_device(self, text_ptr, text_bytes, op, x):
         n = C.c_uint64()

@@ -547,19 +547,24 @@ __device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, 
     return d;
 }
 
-// Walks the tiles [0, tiles) of a range from p_begin, block by block, each
-// tile's text prefetched during the previous one: calls f(tile, desc).
+// Walks the positions [p_begin, pos_end) in tiles, block by block, each
+// tile's text prefetched during the previous one: calls f(tile, desc), where
+// tile t covers positions origin + t * TILE onwards and origin is p_begin
+// rounded down to 16 bytes (tiles load aligned words); positions before
+// p_begin are not described.
 template <typename F>
 __device__ __forceinline__ void for_tiles(const uint8_t* text, uint64_t p_begin, uint64_t pos_end, uint64_t text_len, TileLds& s, const F& f)
 {
-    const uint64_t tiles = (pos_end - p_begin + TILE - 1) / TILE;
+    const uint64_t origin = p_begin & ~uint64_t(15);
+    const uint64_t tiles = (pos_end - origin + TILE - 1) / TILE;
     TileWords w{};
     if(blockIdx.x < tiles)
-        load_tile(text, p_begin + blockIdx.x * uint64_t(TILE), w);
+        load_tile(text, origin + blockIdx.x * uint64_t(TILE), w);
     for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
         const uint64_t next = tile + gridDim.x;
-        const Desc d = describe_tile(text, p_begin + tile * TILE, pos_end, text_len, s, w, p_begin + next * TILE, next < tiles);
+        Desc d = describe_tile(text, origin + tile * TILE, pos_end, text_len, s, w, origin + next * TILE, next < tiles);
+        d.valid &= (origin + tile * TILE + threadIdx.x >= p_begin);
         f(tile, d);
     }
 }

@@ -115,14 +115,14 @@ __device__ __forceinline__ uint32_t segment_of(const QueryArgs& a, uint64_t tile
 }
 
 template <bool MAP, int MODE>
-__global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_end, QueryArgs a, Counters* cnt)
+__global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_begin, uint64_t pos_end, QueryArgs a, Counters* cnt)
 {
     __shared__ TileLds s;
     __shared__ uint64_t red[TILE / 64];
     uint64_t total = 0, positions = 0;
-    for_tiles(text, 0, pos_end, pos_end, s, [&](uint64_t tile, const Desc& d)
+    for_tiles(text, pos_begin, pos_end, pos_end, s, [&](uint64_t tile, const Desc& d)
     {
-        const uint64_t p = tile * TILE + threadIdx.x;
+        const uint64_t p = (pos_begin & ~uint64_t(15)) + tile * TILE + threadIdx.x;
         uint64_t v = 0;
         bool hit = false;
         if(d.valid)
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ t
     __shared__ TileLds s;
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t tile, const Desc& d)
     {
-        const uint64_t p = pos_begin + tile * TILE + threadIdx.x;
+        const uint64_t p = (pos_begin & ~uint64_t(15)) + tile * TILE + threadIdx.x;
         if(p < pos_end)
         {
             const uint64_t i = p - pos_begin;

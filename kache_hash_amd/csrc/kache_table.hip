@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <new>
@@ -277,6 +278,13 @@ struct kache_table
             (void)hipHostFree(h_cnt), h_cnt = nullptr;
         for(auto e : ev_pool)
             (void)hipEventDestroy(e);
+        if(d_ring)
+            (void)hipFree(d_ring), d_ring = nullptr;
+        if(h_ring)
+            (void)hipHostFree(h_ring), h_ring = nullptr;
+        for(auto& e : ring_ev)
+            if(e)
+                (void)hipEventDestroy(e), e = nullptr;
         ev_pool.clear();
         if(stream)
             (void)hipStreamDestroy(stream), stream = nullptr;
@@ -593,46 +601,50 @@ struct kache_table
     // --- Inserts ---
 
     template <bool MAP, uint32_t OP, bool OR>
-    void launch_insert(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
+    void launch_insert(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr, Counters* cnt) const
     {
         const TableView T = view();
-        const uint64_t tiles = (e - b + TILE - 1) / TILE;
-        hipLaunchKernelGGL((k_insert<MAP, OP, OR>), dim3(grid_for(k_insert<MAP, OP, OR>, tiles)), dim3(TILE), 0, stream, T, text, text_len, b, e, x, xr, d_cnt);
+        const uint64_t tiles = (e - (b & ~uint64_t(15)) + TILE - 1) / TILE;
+        hipLaunchKernelGGL((k_insert<MAP, OP, OR>), dim3(grid_for(k_insert<MAP, OP, OR>, tiles)), dim3(TILE), 0, stream, T, text, text_len, b, e, x, xr, cnt);
     }
 
-    void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
+    void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr,
+                         Counters* cnt = nullptr) const
     {
+        if(cnt == nullptr)
+            cnt = d_cnt;
         if(!map)
-            return launch_insert<false, OP_KEEP, false>(text, text_len, b, e, 0, 0);
+            return launch_insert<false, OP_KEEP, false>(text, text_len, b, e, 0, 0, cnt);
         if(oriented)
             switch(op)
             {
-            case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, true>(text, text_len, b, e, x, xr);
-            case OP_ADD:    return launch_insert<true, OP_ADD, true>(text, text_len, b, e, x, xr);
-            case OP_OR:     return launch_insert<true, OP_OR, true>(text, text_len, b, e, x, xr);
-            default:        return launch_insert<true, OP_MAX, true>(text, text_len, b, e, x, xr);
+            case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, true>(text, text_len, b, e, x, xr, cnt);
+            case OP_ADD:    return launch_insert<true, OP_ADD, true>(text, text_len, b, e, x, xr, cnt);
+            case OP_OR:     return launch_insert<true, OP_OR, true>(text, text_len, b, e, x, xr, cnt);
+            default:        return launch_insert<true, OP_MAX, true>(text, text_len, b, e, x, xr, cnt);
             }
         switch(op)
         {
-        case OP_KEEP:   return launch_insert<true, OP_KEEP, false>(text, text_len, b, e, x, 0);
-        case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, false>(text, text_len, b, e, x, 0);
-        case OP_ADD:    return launch_insert<true, OP_ADD, false>(text, text_len, b, e, x, 0);
-        case OP_OR:     return launch_insert<true, OP_OR, false>(text, text_len, b, e, x, 0);
-        case OP_MIN:    return launch_insert<true, OP_MIN, false>(text, text_len, b, e, x, 0);
-        case OP_MAX:    return launch_insert<true, OP_MAX, false>(text, text_len, b, e, x, 0);
-        default:        return launch_insert<true, OP_ADD_WRAP, false>(text, text_len, b, e, x, 0);
+        case OP_KEEP:   return launch_insert<true, OP_KEEP, false>(text, text_len, b, e, x, 0, cnt);
+        case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, false>(text, text_len, b, e, x, 0, cnt);
+        case OP_ADD:    return launch_insert<true, OP_ADD, false>(text, text_len, b, e, x, 0, cnt);
+        case OP_OR:     return launch_insert<true, OP_OR, false>(text, text_len, b, e, x, 0, cnt);
+        case OP_MIN:    return launch_insert<true, OP_MIN, false>(text, text_len, b, e, x, 0, cnt);
+        case OP_MAX:    return launch_insert<true, OP_MAX, false>(text, text_len, b, e, x, 0, cnt);
+        default:        return launch_insert<true, OP_ADD_WRAP, false>(text, text_len, b, e, x, 0, cnt);
         }
     }
 
     // Inserts the positions of a device text, in launches of at most
     // LAUNCH_POS positions, growing the overflow table ahead of each so it
     // cannot fill, and the table after each as the schedule says.
-    uint64_t insert_text(CallSched& s, uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t x, uint64_t xr)
+    uint64_t insert_text(CallSched& s, uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t x, uint64_t xr,
+                         uint64_t first = 0)
     {
         uint64_t inserted = 0;
-        for(uint64_t b = 0; b < text_len; b += LAUNCH_POS)
+        for(uint64_t b = first; b < text_len; b = (b & ~(LAUNCH_POS - 1)) + LAUNCH_POS)
         {
-            const uint64_t e = std::min(text_len, b + LAUNCH_POS);
+            const uint64_t e = std::min(text_len, (b & ~(LAUNCH_POS - 1)) + LAUNCH_POS);
             ov_room(e - b);
             timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
             const Counters c = read_counters();
@@ -683,6 +695,143 @@ struct kache_table
         return inserted;
     }
 
+    // --- Pipelined batches ---
+
+    // Inserts the batches [bounds[i], bounds[i + 1]) of a device text, each
+    // as one insert call of `user` (its count at out[i]), launching batch
+    // i + 1 before batch i's count is read whenever the keys of all batches
+    // in flight cannot reach the growth threshold nor fill the overflow
+    // table: then no doubling can fall due among them, and the growth
+    // schedule, applied batch by batch as their counts arrive, is exact.
+    uint64_t insert_batches(uint32_t user, const uint8_t* text, const uint64_t* bounds, uint64_t n, uint32_t op, bool oriented,
+                            uint64_t x, uint64_t xr, uint64_t* out)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        ensure_ring();
+        struct Flight { uint64_t batch; int slot; uint64_t positions; };
+        std::deque<Flight> q;
+        uint64_t in_flight = 0, total = 0, next_slot = 0;
+
+        const auto resolve = [&]()
+        {
+            const Flight f = q.front();
+            q.pop_front();
+            HIP(hipEventSynchronize(ring_ev[f.slot]));
+            const Counters c = h_ring[f.slot];
+            in_flight -= f.positions;
+            if(c.failed)
+                throw hip_failure("an insert found no room in the overflow table");
+            size += c.inserted, ov_count += c.ov_inserted, total += c.inserted;
+            if(out)
+                out[f.batch] = c.inserted;
+            CallSched s = sched_begin(user);
+            const uint32_t d = sched_add(s, c.inserted);
+            sched_end(s, user);
+            if(d)
+            {
+                sync();
+                grow(d);
+            }
+        };
+
+        for(uint64_t i = 0; i < n; ++i)
+        {
+            const uint64_t b = bounds[i], e = bounds[i + 1];
+            const uint64_t pos = e - b;
+            if(pos > LAUNCH_POS)
+            {
+                // A batch too large for one launch goes alone, as a call.
+                while(!q.empty())
+                    resolve();
+                sync();
+                CallSched s = sched_begin(user);
+                const uint64_t c = insert_text(s, op, oriented, text + (b & ~uint64_t(15)), e - (b & ~uint64_t(15)), x, xr, b & 15);
+                sched_end(s, user);
+                total += c;
+                if(out)
+                    out[i] = c;
+                continue;
+            }
+            const auto fits_now = [&]()
+            {
+                const uint64_t keys_bound = size + in_flight + pos;
+                return q.size() < RING && keys_bound < resize_th && ov_count + in_flight + pos + 1 <= ov_slots &&
+                       static_cast<double>(ov_count + in_flight + pos) < 0.75 * static_cast<double>(ov_slots);
+            };
+            while(!q.empty() && !fits_now())
+                resolve();
+            if(q.empty())
+            {
+                sync();
+                ov_room(pos);
+            }
+            const int slot = static_cast<int>(next_slot++ % RING);
+            timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot); });
+            HIP(hipMemcpyAsync(h_ring + slot, d_ring + slot, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+            hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_ring + slot), sizeof(Counters) / 8);
+            HIP(hipGetLastError());
+            HIP(hipEventRecord(ring_ev[slot], stream));
+            q.push_back({i, slot, pos});
+            in_flight += pos;
+        }
+        while(!q.empty())
+            resolve();
+        sync();
+        return total;
+    }
+
+    // Lookups of the batches of a device text, all launched back to back:
+    // hits and k-mer positions of batch i at hits[i], positions[i].
+    void count_batches(const uint8_t* text, const uint64_t* bounds, uint64_t n, uint64_t* hits, uint64_t* positions) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        DevBuf cnt;
+        cnt.reserve(n * sizeof(Counters));
+        hipLaunchKernelGGL(k_zero64, dim3(std::max<uint64_t>(1, std::min<uint64_t>((n * sizeof(Counters) / 8 + 255) / 256, 1024))), dim3(256), 0, stream,
+                           cnt.as<uint64_t>(), n * sizeof(Counters) / 8);
+        HIP(hipGetLastError());
+        for(uint64_t i = 0; i < n; ++i)
+        {
+            const uint64_t b = bounds[i], e = bounds[i + 1];
+            QueryArgs a{};
+            const uint64_t tiles = (e - (b & ~uint64_t(15)) + TILE - 1) / TILE;
+            Counters* c = cnt.as<Counters>() + i;
+            timed("query", [&]{
+                if(map)
+                    hipLaunchKernelGGL((k_query<true, Q_TOTAL>), dim3(grid_for(k_query<true, Q_TOTAL>, tiles)), dim3(TILE), 0, stream, view(), text, b, e, a, c);
+                else
+                    hipLaunchKernelGGL((k_query<false, Q_TOTAL>), dim3(grid_for(k_query<false, Q_TOTAL>, tiles)), dim3(TILE), 0, stream, view(), text, b, e, a, c);
+            });
+        }
+        std::vector<Counters> h(n);
+        HIP(hipMemcpyAsync(h.data(), cnt.p, n * sizeof(Counters), hipMemcpyDeviceToHost, stream));
+        sync();
+        for(uint64_t i = 0; i < n; ++i)
+        {
+            if(hits)
+                hits[i] = h[i].present;
+            if(positions)
+                positions[i] = h[i].positions;
+        }
+    }
+
+    static constexpr int RING = 16;         // counter slots of batches in flight
+    mutable Counters* d_ring = nullptr;
+    mutable Counters* h_ring = nullptr;
+    mutable hipEvent_t ring_ev[RING] = {};
+
+    void ensure_ring() const
+    {
+        if(d_ring)
+            return;
+        d_ring = dev_alloc<Counters>(RING);
+        HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ring), RING * sizeof(Counters), hipHostMallocDefault));
+        for(auto& e : ring_ev)
+            HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * sizeof(Counters) / 8);
+        HIP(hipGetLastError());
+    }
+
     // --- Updates ---
 
     template <uint32_t OP>
@@ -728,9 +877,9 @@ struct kache_table
         const uint64_t tiles = (text_len + TILE - 1) / TILE;
         timed("query", [&]{
             if(map)
-                hipLaunchKernelGGL((k_query<true, MODE>), dim3(grid_for(k_query<true, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, a, d_cnt);
+                hipLaunchKernelGGL((k_query<true, MODE>), dim3(grid_for(k_query<true, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
             else
-                hipLaunchKernelGGL((k_query<false, MODE>), dim3(grid_for(k_query<false, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, a, d_cnt);
+                hipLaunchKernelGGL((k_query<false, MODE>), dim3(grid_for(k_query<false, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
         });
     }
 
@@ -1238,6 +1387,37 @@ int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* 
         const uint64_t c = text_bytes ? t->insert_device(user, reinterpret_cast<const uint8_t*>(text), text_bytes, op, true, x_fwd, x_rev) : 0;
         if(inserted != nullptr)
             *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, const uint64_t* bounds, uint64_t n_batches, uint32_t op,
+                             uint64_t x, uint64_t* inserted) noexcept
+{
+    if(t == nullptr || !registered(t, user) || !device_text_ok(text, n_batches ? bounds[n_batches] : 0) || (n_batches > 0 && bounds == nullptr) ||
+       (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
+        return KACHE_EINVAL;
+    for(uint64_t i = 0; i < n_batches; ++i)
+        if(bounds[i] > bounds[i + 1])
+            return KACHE_EINVAL;
+    return guard([&]()
+    {
+        t->insert_batches(user, reinterpret_cast<const uint8_t*>(text), bounds, n_batches, t->map ? op : 0, false, t->map ? x : 0, 0, inserted);
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_count_batches(const kache_table* t, const char* text, const uint64_t* bounds, uint64_t n_batches, uint64_t* hits, uint64_t* positions) noexcept
+{
+    if(t == nullptr || hits == nullptr || !device_text_ok(text, n_batches ? bounds[n_batches] : 0) || (n_batches > 0 && bounds == nullptr))
+        return KACHE_EINVAL;
+    for(uint64_t i = 0; i < n_batches; ++i)
+        if(bounds[i] > bounds[i + 1])
+            return KACHE_EINVAL;
+    return guard([&]()
+    {
+        if(n_batches)
+            t->count_batches(reinterpret_cast<const uint8_t*>(text), bounds, n_batches, hits, positions);
         return static_cast<int>(KACHE_OK);
     });
 }

@@ -301,3 +301,39 @@ def test_synthetic_reads_match_host_generator(api):
     assert np.isin(upper, np.frombuffer(b"ACGTN", np.uint8)).all()
     frac_n = (upper == ord("N")).mean()
     assert 0.0005 < frac_n < 0.002
+
+
+@pytest.mark.parametrize("expected", [10, 2_000_000])
+def test_pipelined_batches_equal_one_call_per_batch(api, expected):
+    """kache_amd_insert_batches against one kache_amd_insert_device call per
+    batch: per-batch counts, contents and capacity, from a tiny table that
+    doubles many times (the pipeline drains before each doubling) and a
+    presized one; batch starts at any byte offset."""
+    g = G.genome(300_000, 81)
+    R = G.reads(g, 40_000, 150, 82, sub=0.01, nrate=0.001, lower=0.1)
+    rng = np.random.default_rng(83)
+    cuts = np.sort(rng.choice(np.arange(1, len(R)), 60, replace=False))
+    batches = np.split(np.arange(len(R)), cuts)
+    text = b"".join(r + b"\n" for r in R)
+    ends = np.cumsum([len(r) + 1 for r in R])
+    bounds = np.array([0] + [int(ends[b[-1]]) for b in batches], np.uint64)
+    buf, n = capi.device_text(api, R)
+    with capi.Table(api, 4, expected, 0.8, False, 4) as a, capi.Table(api, 4, expected, 0.8, False, 4) as b:
+        ua, ub = a.register(), b.register()
+        got = a.insert_batches(buf.ptr, bounds, capi.ADD, 2, ua)
+        ref = []
+        for i in range(len(bounds) - 1):
+            lo, hi = int(bounds[i]), int(bounds[i + 1])
+            sub, m = capi.device_text(api, [text[lo:hi]])
+            ref.append(b.insert_device(sub.ptr, m, capi.ADD, 2, ub))
+        assert got.tolist() == ref
+        ka, va, _ = a.for_each()
+        kb, vb, _ = b.for_each()
+        assert (ka == kb).all() and (va == vb).all()
+        assert a.stats()["capacity"] == b.stats()["capacity"]
+        hits, pos = a.count_batches(buf.ptr, bounds)
+        for i in range(len(bounds) - 1):
+            lo, hi = int(bounds[i]), int(bounds[i + 1])
+            sub, m = capi.device_text(api, [text[lo:hi]])
+            h2, p2 = b.count_device(sub.ptr, m, positions=True)
+            assert (int(hits[i]), int(pos[i])) == (h2, p2) and h2 == p2
