@@ -73,6 +73,7 @@ typedef struct kache_amd_info
     uint64_t overflow_keys;
     uint64_t resizes;
     uint64_t device_bytes;      /* bucket store + overflow table */
+    uint64_t value_bytes;       /* 0 for a set */
 } kache_amd_info;
 int kache_amd_get_info(const kache_table* t, kache_amd_info* out) KACHE_NOEXCEPT;
 
@@ -94,6 +95,78 @@ int kache_amd_synth_reads(char* out, uint64_t n_reads, uint64_t read_len, uint64
 int kache_amd_host_synth_reads(char* out, uint64_t first_read, uint64_t n_reads, uint64_t read_len, uint64_t genome_seed,
                                uint64_t genome_len, uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
 const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
+
+/* Serving loop over a resident table.  Requests are submitted from any
+ * thread and run by one worker thread of the server, in waves: the worker
+ * takes the pending requests in arrival order, as many as fit `wave_bytes`
+ * of staged sequence (a larger request runs as a wave of its own) and at
+ * most `max_wave_requests`, and runs the wave in three phases: all its reads
+ * (COUNT and LOOKUP) merged into one call each, then its updates one by one,
+ * then its upserts, one pipelined batch call (kache_amd_insert_batches) per
+ * run of equal op and operand, a batch per request.  Requests arriving while
+ * a wave runs join the next one.  Each request's results are those of
+ * running all requests alone, one after another, in served order: by wave,
+ * then phase (reads 0, updates 1, upserts 2), then ticket.  With
+ * max_wave_requests = 1 every request is one plain C-ABI call.
+ *
+ * The request's arrays (sequences, outputs) must stay valid until its wait
+ * returns.  Upserts run as `user` (from kache_register); while a server
+ * runs, other callers of the table keep the C ABI's threading contract with
+ * respect to its calls. */
+#define KACHE_AMD_REQ_COUNT 0   /* kache_count: counts[i] per sequence */
+#define KACHE_AMD_REQ_LOOKUP 1  /* kache_lookup: found / values / fwd per position (values, fwd may be NULL) */
+#define KACHE_AMD_REQ_UPDATE 2  /* kache_update(op, x): *counts = present positions (counts may be NULL) */
+#define KACHE_AMD_REQ_INSERT 3  /* kache_insert(op, x): *counts = new keys (counts may be NULL) */
+
+typedef struct kache_amd_request
+{
+    uint32_t kind;
+    uint32_t op;
+    uint64_t x;
+    const char* const* seqs;
+    const size_t* lens;
+    size_t n;
+    uint64_t* counts;
+    uint8_t* found;
+    void* values;
+    uint8_t* fwd;
+} kache_amd_request;
+
+typedef struct kache_amd_request_result
+{
+    int status;                 /* the call's status */
+    uint32_t phase;             /* 0 reads, 1 updates, 2 upserts */
+    uint64_t ticket;
+    uint64_t wave;              /* 1, 2, ... */
+    double submit_s;            /* kache_amd_clock() at submission, start of its wave, completion */
+    double start_s;
+    double done_s;
+} kache_amd_request_result;
+
+typedef struct kache_amd_server_stats
+{
+    uint64_t waves;
+    uint64_t requests;
+    uint64_t max_wave_requests;
+    uint64_t max_wave_bytes;
+} kache_amd_server_stats;
+
+typedef struct kache_amd_server kache_amd_server;
+
+int kache_amd_server_new(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_wave_requests, kache_amd_server** out) KACHE_NOEXCEPT;
+/* Runs the requests still pending, then stops the worker. */
+void kache_amd_server_free(kache_amd_server* s) KACHE_NOEXCEPT;
+int kache_amd_server_submit(kache_amd_server* s, const kache_amd_request* r, uint64_t* ticket) KACHE_NOEXCEPT;
+/* Blocks until the request is done; each ticket is waited for once. */
+int kache_amd_server_wait(kache_amd_server* s, uint64_t ticket, kache_amd_request_result* res) KACHE_NOEXCEPT;
+int kache_amd_server_get_stats(kache_amd_server* s, kache_amd_server_stats* out) KACHE_NOEXCEPT;
+/* Open-loop load: submits request i at `arrival_s[i]` seconds (non-decreasing)
+ * after the call starts, from the calling thread, while another thread waits
+ * for them in order; returns when all are done, with results[i] for each. */
+int kache_amd_server_replay(kache_amd_server* s, const kache_amd_request* reqs, const double* arrival_s, size_t n,
+                            kache_amd_request_result* results) KACHE_NOEXCEPT;
+/* Seconds on the steady clock the results use. */
+double kache_amd_clock(void) KACHE_NOEXCEPT;
 
 /* Description of the last failure of a call on this thread. */
 const char* kache_amd_last_error(void) KACHE_NOEXCEPT;

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkache_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kache_table.hip", "kache_synth.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kache_table.hip", "kache_synth.hip", "kache_serve.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("kache_device.hpp", "kache_kernels.hpp")] + [
     os.path.join(ROOT, "include", f) for f in ("kache_capi.h", "kache_amd.h")
 ]

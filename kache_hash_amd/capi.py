@@ -34,7 +34,26 @@ class Stats(C.Structure):
 
 class Info(C.Structure):
     _fields_ = [("buckets", C.c_uint64), ("bucket_bytes", C.c_uint64), ("overflow_slots", C.c_uint64),
-                ("overflow_keys", C.c_uint64), ("resizes", C.c_uint64), ("device_bytes", C.c_uint64)]
+                ("overflow_keys", C.c_uint64), ("resizes", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("value_bytes", C.c_uint64)]
+
+
+REQ_COUNT, REQ_LOOKUP, REQ_UPDATE, REQ_INSERT = range(4)   # KACHE_AMD_REQ_*
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("op", C.c_uint32), ("x", C.c_uint64), ("seqs", C.c_void_p), ("lens", C.c_void_p),
+                ("n", C.c_size_t), ("counts", C.c_void_p), ("found", C.c_void_p), ("values", C.c_void_p), ("fwd", C.c_void_p)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("phase", C.c_uint32), ("ticket", C.c_uint64), ("wave", C.c_uint64),
+                ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("waves", C.c_uint64), ("requests", C.c_uint64), ("max_wave_requests", C.c_uint64),
+                ("max_wave_bytes", C.c_uint64)]
 
 
 KEYS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_size_t)
@@ -85,6 +104,13 @@ AMD_SYMBOLS = {
     "kache_amd_set_blocks_per_cu": (C.c_int, [_P, C.c_uint32]),
     "kache_amd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
     "kache_amd_last_error": (C.c_char_p, []),
+    "kache_amd_server_new": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(_P)]),
+    "kache_amd_server_free": (None, [_P]),
+    "kache_amd_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "kache_amd_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestResult)]),
+    "kache_amd_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "kache_amd_server_replay": (C.c_int, [_P, C.POINTER(Request), _P, C.c_size_t, C.POINTER(RequestResult)]),
+    "kache_amd_clock": (C.c_double, []),
     "kache_amd_host_minimizer_hash": (C.c_uint64, [C.c_uint64]),
     "kache_amd_host_checksum": (C.c_uint8, [C.c_uint64]),
     "kache_amd_host_xxh3": (C.c_uint64, [C.c_uint64, C.c_uint64]),
@@ -373,6 +399,86 @@ class DeviceBuffer:
         out = np.empty(n, np.uint8)
         self.api.check(self.api.lib.kache_amd_copy(out.ctypes.data, self.ptr + offset, n, 0), "kache_amd_copy")
         return out.view(dtype)
+
+
+class Server:
+    """The serving loop of a table (kache_amd_server_*): requests submitted
+    from any thread run in waves on one worker; `user` runs the upserts."""
+
+    def __init__(self, table, user, wave_bytes=64 << 20, max_wave_requests=1 << 20):
+        self.table, self.api = table, table.api
+        h = C.c_void_p()
+        self.api.check(self.api.lib.kache_amd_server_new(table.h, user, wave_bytes, max_wave_requests, C.byref(h)), "kache_amd_server_new")
+        self.h = h
+        self.keep = {}
+
+    def close(self):
+        if self.h:
+            self.api.lib.kache_amd_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def request(self, kind, seqs, op=KEEP, x=0, values=True, fwd=True):
+        """A request and the arrays it writes: (Request, outputs, keep-alive)."""
+        b = as_batch(seqs)
+        r = Request(kind=kind, op=op, x=x, seqs=b.c_seqs, lens=b.c_lens, n=b.n)
+        out = {}
+        if kind == REQ_COUNT:
+            out["counts"] = np.zeros(max(b.n, 1), np.uint64)
+            r.counts = out["counts"].ctypes.data
+        elif kind == REQ_LOOKUP:
+            p = max(b.positions(), 1)
+            out["found"] = np.zeros(p, np.uint8)
+            r.found = out["found"].ctypes.data
+            if values and self.table.width:
+                out["values"] = np.zeros(p * self.table.width, np.uint8)
+                r.values = out["values"].ctypes.data
+            if fwd:
+                out["fwd"] = np.zeros(p, np.uint8)
+                r.fwd = out["fwd"].ctypes.data
+        else:
+            out["counts"] = np.zeros(1, np.uint64)
+            r.counts = out["counts"].ctypes.data
+        return r, out, b
+
+    def submit(self, req):
+        """Submits (Request, outputs, batch) from request(); returns the ticket."""
+        t = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_server_submit(self.h, C.byref(req[0]), C.byref(t)), "kache_amd_server_submit")
+        self.keep[t.value] = req
+        return t.value
+
+    def wait(self, ticket):
+        """Waits for a ticket; returns (result dict, outputs)."""
+        res = RequestResult()
+        self.api.check(self.api.lib.kache_amd_server_wait(self.h, ticket, C.byref(res)), "kache_amd_server_wait")
+        req = self.keep.pop(ticket)
+        return {f: getattr(res, f) for f, _ in RequestResult._fields_}, req[1]
+
+    def replay(self, reqs, arrivals):
+        """Open-loop load: request i submitted at arrivals[i] s; returns the results (a list of dicts)."""
+        n = len(reqs)
+        arr = (Request * max(n, 1))(*[r[0] for r in reqs])
+        times = np.ascontiguousarray(arrivals, dtype=np.float64)
+        out = (RequestResult * max(n, 1))()
+        self.api.check(self.api.lib.kache_amd_server_replay(self.h, arr, times.ctypes.data, n, out), "kache_amd_server_replay")
+        return [{f: getattr(out[i], f) for f, _ in RequestResult._fields_} for i in range(n)]
+
+    def stats(self):
+        s = ServerStats()
+        self.api.check(self.api.lib.kache_amd_server_get_stats(self.h, C.byref(s)), "kache_amd_server_get_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
 
 
 def device_text(api, seqs):

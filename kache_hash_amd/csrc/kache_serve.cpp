@@ -1,0 +1,460 @@
+// Serving loop over a resident table (kache_amd_server_* in include/kache_amd.h).
+// Requests (per-sequence counts, per-position lookups, present-only updates,
+// upserts) arrive from any thread; one worker admits them in arrival order
+// into waves, as many as fit the wave's staging budget in HBM, and runs each
+// wave as: its reads merged into one call per kind (one staging copy, one
+// kernel), its updates, then its upserts, packed into one device text and
+// inserted by one pipelined batch call per run of equal op and operand (each
+// request one batch, so its count of new keys is its own).  Requests arriving
+// while a wave runs join the next one.  Each request sees exactly what a
+// serial run in served order (wave, phase, ticket) gives it.
+//
+// It uses only the C ABI and the device extensions, like any other client.
+
+#include "kache_amd.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace
+{
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req
+{
+    kache_amd_request r;
+    uint64_t ticket = 0;
+    uint64_t bytes = 0;     // staged bytes: its sequences and separators
+    int status = KACHE_OK;
+    bool done = false;
+    uint32_t phase = 0;
+    uint64_t wave = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+uint64_t staged_bytes(const kache_amd_request& r)
+{
+    uint64_t b = 0;
+    for(size_t i = 0; i < r.n; ++i)
+        b += r.lens[i] + 1;
+    return b;
+}
+
+// Lookup positions of a request: sum over its sequences of max(0, len - 30)
+// (capi/Kmer_Table.hpp:163-190).
+uint64_t positions_of(const kache_amd_request& r)
+{
+    uint64_t p = 0;
+    for(size_t i = 0; i < r.n; ++i)
+        p += r.lens[i] >= 31 ? r.lens[i] - 30 : 0;
+    return p;
+}
+
+uint32_t phase_of(uint32_t kind)
+{
+    return kind == KACHE_AMD_REQ_UPDATE ? 1 : kind == KACHE_AMD_REQ_INSERT ? 2 : 0;
+}
+
+}
+
+struct kache_amd_server
+{
+    kache_table* t;
+    uint32_t user;
+    uint64_t wave_bytes;
+    uint32_t max_wave_requests;
+    uint32_t width;
+
+    std::mutex mu;
+    std::condition_variable cv_in, cv_done;
+    std::deque<std::shared_ptr<Req>> pending;
+    std::map<uint64_t, std::shared_ptr<Req>> live;
+    uint64_t next_ticket = 1;
+    bool stopping = false;
+    std::thread worker;
+
+    // Staging of a wave's upserts: host bytes, their device copy.
+    std::vector<char> h_text;
+    void* d_text = nullptr;
+    uint64_t d_bytes = 0;
+
+    kache_amd_server_stats stats{};
+
+    kache_amd_server(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_req, uint32_t width)
+        : t(t), user(user), wave_bytes(wave_bytes), max_wave_requests(max_req), width(width)
+    {
+        worker = std::thread([this]{ loop(); });
+    }
+
+    ~kache_amd_server()
+    {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stopping = true;
+        }
+        cv_in.notify_all();
+        worker.join();
+        if(d_text)
+            kache_amd_device_free(d_text);
+    }
+
+    void loop()
+    {
+        while(true)
+        {
+            std::vector<std::shared_ptr<Req>> wave;
+            uint64_t wave_id = 0;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_in.wait(lock, [&]{ return stopping || !pending.empty(); });
+                if(pending.empty())
+                    return;
+                uint64_t bytes = 0;
+                while(!pending.empty() && wave.size() < max_wave_requests &&
+                      (wave.empty() || bytes + pending.front()->bytes <= wave_bytes))
+                {
+                    bytes += pending.front()->bytes;
+                    wave.push_back(pending.front());
+                    pending.pop_front();
+                }
+                wave_id = ++stats.waves;
+                stats.requests += wave.size();
+                stats.max_wave_requests = std::max<uint64_t>(stats.max_wave_requests, wave.size());
+                stats.max_wave_bytes = std::max(stats.max_wave_bytes, bytes);
+            }
+            const double t0 = now_s();
+            for(auto& q : wave)
+                q->t_start = t0, q->wave = wave_id, q->phase = phase_of(q->r.kind);
+            run(wave);
+            const double t1 = now_s();
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                for(auto& q : wave)
+                    q->t_done = t1, q->done = true;
+            }
+            cv_done.notify_all();
+        }
+    }
+
+    void run(std::vector<std::shared_ptr<Req>>& wave)
+    {
+        std::vector<Req*> counts, lookups, updates, inserts;
+        for(auto& q : wave)
+            switch(q->r.kind)
+            {
+            case KACHE_AMD_REQ_COUNT: counts.push_back(q.get()); break;
+            case KACHE_AMD_REQ_LOOKUP: lookups.push_back(q.get()); break;
+            case KACHE_AMD_REQ_UPDATE: updates.push_back(q.get()); break;
+            default: inserts.push_back(q.get()); break;
+            }
+        run_counts(counts);
+        run_lookups(lookups);
+        for(Req* q : updates)
+            q->status = kache_update(t, q->r.seqs, q->r.lens, q->r.n, q->r.op, q->r.x, q->r.counts);
+        for(size_t a = 0; a < inserts.size();)
+        {
+            size_t b = a + 1;
+            while(b < inserts.size() && inserts[b]->r.op == inserts[a]->r.op && inserts[b]->r.x == inserts[a]->r.x)
+                b++;
+            run_inserts(inserts, a, b);
+            a = b;
+        }
+    }
+
+    // All sequences of the requests, in order, as one call's arrays.
+    static void merge(const std::vector<Req*>& rs, std::vector<const char*>& seqs, std::vector<size_t>& lens)
+    {
+        for(Req* q : rs)
+        {
+            seqs.insert(seqs.end(), q->r.seqs, q->r.seqs + q->r.n);
+            lens.insert(lens.end(), q->r.lens, q->r.lens + q->r.n);
+        }
+    }
+
+    void run_counts(const std::vector<Req*>& rs)
+    {
+        if(rs.size() == 1)
+        {
+            rs[0]->status = kache_count(t, rs[0]->r.seqs, rs[0]->r.lens, rs[0]->r.n, rs[0]->r.counts);
+            return;
+        }
+        if(rs.empty())
+            return;
+        std::vector<const char*> seqs;
+        std::vector<size_t> lens;
+        merge(rs, seqs, lens);
+        std::vector<uint64_t> out(std::max<size_t>(seqs.size(), 1));
+        const int st = kache_count(t, seqs.data(), lens.data(), seqs.size(), out.data());
+        size_t o = 0;
+        for(Req* q : rs)
+        {
+            q->status = st;
+            if(st == KACHE_OK && q->r.n)
+                std::memcpy(q->r.counts, out.data() + o, q->r.n * sizeof(uint64_t));
+            o += q->r.n;
+        }
+    }
+
+    void run_lookups(const std::vector<Req*>& rs)
+    {
+        if(rs.size() == 1)
+        {
+            const auto& r = rs[0]->r;
+            rs[0]->status = kache_lookup(t, r.seqs, r.lens, r.n, r.found, r.values, r.fwd);
+            return;
+        }
+        if(rs.empty())
+            return;
+        std::vector<const char*> seqs;
+        std::vector<size_t> lens;
+        merge(rs, seqs, lens);
+        uint64_t total = 0;
+        bool want_values = false, want_fwd = false;
+        for(Req* q : rs)
+            total += positions_of(q->r), want_values |= q->r.values != nullptr, want_fwd |= q->r.fwd != nullptr;
+        std::vector<uint8_t> found(std::max<uint64_t>(total, 1));
+        std::vector<uint8_t> values(want_values && width ? std::max<uint64_t>(total, 1) * width : 0);
+        std::vector<uint8_t> fwd(want_fwd ? std::max<uint64_t>(total, 1) : 0);
+        const int st = kache_lookup(t, seqs.data(), lens.data(), seqs.size(), found.data(), values.empty() ? nullptr : values.data(),
+                                    fwd.empty() ? nullptr : fwd.data());
+        uint64_t o = 0;
+        for(Req* q : rs)
+        {
+            const uint64_t p = positions_of(q->r);
+            q->status = st;
+            if(st == KACHE_OK && p)
+            {
+                std::memcpy(q->r.found, found.data() + o, p);
+                if(q->r.values && width)
+                    std::memcpy(q->r.values, values.data() + o * width, p * width);
+                if(q->r.fwd)
+                    std::memcpy(q->r.fwd, fwd.data() + o, p);
+            }
+            o += p;
+        }
+    }
+
+    void run_inserts(const std::vector<Req*>& ins, size_t a, size_t b)
+    {
+        if(b - a == 1)
+        {
+            const auto& r = ins[a]->r;
+            ins[a]->status = kache_insert(t, user, r.seqs, r.lens, r.n, r.op, r.x, r.counts);
+            return;
+        }
+        std::vector<uint64_t> bounds{0};
+        uint64_t bytes = 0;
+        for(size_t i = a; i < b; ++i)
+            bytes += ins[i]->bytes;
+        h_text.resize(bytes + KACHE_AMD_TEXT_PAD);
+        uint64_t o = 0;
+        for(size_t i = a; i < b; ++i)
+        {
+            const auto& r = ins[i]->r;
+            for(size_t j = 0; j < r.n; ++j)
+            {
+                if(r.lens[j])
+                    std::memcpy(h_text.data() + o, r.seqs[j], r.lens[j]);
+                o += r.lens[j];
+                h_text[o++] = '\n';
+            }
+            bounds.push_back(o);
+        }
+        std::memset(h_text.data() + o, '\n', KACHE_AMD_TEXT_PAD);
+        int st = KACHE_OK;
+        if(bytes + KACHE_AMD_TEXT_PAD > d_bytes)
+        {
+            if(d_text)
+                kache_amd_device_free(d_text), d_text = nullptr, d_bytes = 0;
+            const uint64_t want = std::max<uint64_t>(bytes, wave_bytes) + KACHE_AMD_TEXT_PAD;
+            st = kache_amd_device_alloc(want, &d_text);
+            if(st == KACHE_OK)
+                d_bytes = want;
+        }
+        std::vector<uint64_t> out(b - a);
+        if(st == KACHE_OK)
+            st = kache_amd_copy(d_text, h_text.data(), bytes + KACHE_AMD_TEXT_PAD, 1);
+        if(st == KACHE_OK)
+            st = kache_amd_insert_batches(t, user, static_cast<const char*>(d_text), bounds.data(), b - a, ins[a]->r.op, ins[a]->r.x, out.data());
+        for(size_t i = a; i < b; ++i)
+        {
+            ins[i]->status = st;
+            if(st == KACHE_OK && ins[i]->r.counts)
+                *ins[i]->r.counts = out[i - a];
+        }
+    }
+};
+
+extern "C"
+{
+
+int kache_amd_server_new(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_wave_requests, kache_amd_server** out) noexcept
+{
+    if(t == nullptr || out == nullptr || wave_bytes == 0 || max_wave_requests == 0)
+        return KACHE_EINVAL;
+    kache_amd_info info{};
+    const int st = kache_amd_get_info(t, &info);
+    if(st != KACHE_OK)
+        return st;
+    try
+    {
+        *out = new kache_amd_server(t, user, wave_bytes, max_wave_requests, static_cast<uint32_t>(info.value_bytes));
+        return KACHE_OK;
+    }
+    catch(...)
+    {
+        return KACHE_EINTERNAL;
+    }
+}
+
+void kache_amd_server_free(kache_amd_server* s) noexcept
+{
+    delete s;
+}
+
+int kache_amd_server_submit(kache_amd_server* s, const kache_amd_request* r, uint64_t* ticket) noexcept
+{
+    if(s == nullptr || r == nullptr || ticket == nullptr || r->kind > KACHE_AMD_REQ_INSERT || (r->n > 0 && (!r->seqs || !r->lens)))
+        return KACHE_EINVAL;
+    if((r->kind == KACHE_AMD_REQ_COUNT && r->n > 0 && !r->counts) || (r->kind == KACHE_AMD_REQ_LOOKUP && !r->found))
+        return KACHE_EINVAL;
+    for(size_t i = 0; i < r->n; ++i)
+        if(r->seqs[i] == nullptr && r->lens[i] > 0)
+            return KACHE_EINVAL;
+    try
+    {
+        auto q = std::make_shared<Req>();
+        q->r = *r;
+        q->bytes = staged_bytes(*r);
+        q->t_submit = now_s();
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            q->ticket = s->next_ticket++;
+            s->live[q->ticket] = q;
+            s->pending.push_back(q);
+        }
+        s->cv_in.notify_one();
+        *ticket = q->ticket;
+        return KACHE_OK;
+    }
+    catch(...)
+    {
+        return KACHE_EINTERNAL;
+    }
+}
+
+int kache_amd_server_wait(kache_amd_server* s, uint64_t ticket, kache_amd_request_result* res) noexcept
+{
+    if(s == nullptr)
+        return KACHE_EINVAL;
+    std::unique_lock<std::mutex> lock(s->mu);
+    const auto it = s->live.find(ticket);
+    if(it == s->live.end())
+        return KACHE_EINVAL;
+    auto q = it->second;
+    s->cv_done.wait(lock, [&]{ return q->done; });
+    s->live.erase(ticket);
+    if(res)
+    {
+        res->status = q->status;
+        res->phase = q->phase;
+        res->ticket = q->ticket;
+        res->wave = q->wave;
+        res->submit_s = q->t_submit;
+        res->start_s = q->t_start;
+        res->done_s = q->t_done;
+    }
+    return KACHE_OK;
+}
+
+int kache_amd_server_get_stats(kache_amd_server* s, kache_amd_server_stats* out) noexcept
+{
+    if(s == nullptr || out == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->stats;
+    return KACHE_OK;
+}
+
+int kache_amd_server_replay(kache_amd_server* s, const kache_amd_request* reqs, const double* arrival_s, size_t n,
+                            kache_amd_request_result* results) noexcept
+{
+    if(s == nullptr || (n > 0 && (reqs == nullptr || arrival_s == nullptr || results == nullptr)))
+        return KACHE_EINVAL;
+    for(size_t i = 1; i < n; ++i)
+        if(!(arrival_s[i] >= arrival_s[i - 1]))
+            return KACHE_EINVAL;
+    try
+    {
+        std::vector<uint64_t> tickets(n);
+        std::vector<int> wait_status(n, KACHE_OK);
+        std::mutex m;
+        std::condition_variable cv;
+        size_t submitted = 0;
+        bool failed = false;
+        std::thread waiter([&]
+        {
+            for(size_t i = 0; i < n; ++i)
+            {
+                uint64_t ticket = 0;
+                {
+                    std::unique_lock<std::mutex> lock(m);
+                    cv.wait(lock, [&]{ return submitted > i || failed; });
+                    if(submitted <= i)
+                        return;
+                    ticket = tickets[i];
+                }
+                wait_status[i] = kache_amd_server_wait(s, ticket, results + i);
+            }
+        });
+        const double t0 = now_s();
+        int st = KACHE_OK;
+        for(size_t i = 0; i < n && st == KACHE_OK; ++i)
+        {
+            // Sleep to within 200 us of the arrival, then spin: timer wake-ups are coarse.
+            const double due = t0 + arrival_s[i];
+            for(double left = due - now_s(); left > 0; left = due - now_s())
+                if(left > 200e-6)
+                    std::this_thread::sleep_for(std::chrono::duration<double>(left - 150e-6));
+            uint64_t ticket = 0;
+            st = kache_amd_server_submit(s, reqs + i, &ticket);
+            std::lock_guard<std::mutex> lock(m);
+            if(st == KACHE_OK)
+                tickets[i] = ticket, submitted = i + 1;
+            else
+                failed = true;
+            cv.notify_one();
+        }
+        waiter.join();
+        if(st != KACHE_OK)
+            return st;
+        for(size_t i = 0; i < n; ++i)
+            if(wait_status[i] != KACHE_OK)
+                return wait_status[i];
+        return KACHE_OK;
+    }
+    catch(...)
+    {
+        return KACHE_EINTERNAL;
+    }
+}
+
+double kache_amd_clock(void) noexcept
+{
+    return now_s();
+}
+
+}

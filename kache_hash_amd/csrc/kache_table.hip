@@ -1517,6 +1517,7 @@ int kache_amd_get_info(const kache_table* t, kache_amd_info* out) noexcept
     out->overflow_keys = t->ov_count;
     out->resizes = t->resizes;
     out->device_bytes = t->cap * t->bucket_bytes() + t->ov_slots * t->slot_words() * 8;
+    out->value_bytes = t->value_bytes;
     return KACHE_OK;
 }
 

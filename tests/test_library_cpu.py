@@ -134,3 +134,18 @@ def test_device_describer_on_host_matches_reference_vectors(api):
     assert valid[pos].all()
     assert (key[pos] == d["key"]).all() and (h[pos] == d["h"]).all()
     assert (c[pos] == d["c_wang"]).all() and (fwd[pos] == d["fwd"]).all()
+
+
+def test_server_rejects_invalid_arguments(api):
+    """The serving loop validates before it starts a worker or touches a device."""
+    L = api.lib
+    E = capi.KACHE_EINVAL
+    h = C.c_void_p()
+    tk = C.c_uint64()
+    r = capi.Request(kind=capi.REQ_COUNT)
+    assert L.kache_amd_server_new(None, 0, 1 << 20, 16, C.byref(h)) == E and not h.value
+    assert L.kache_amd_server_submit(None, C.byref(r), C.byref(tk)) == E
+    assert L.kache_amd_server_wait(None, 1, None) == E
+    assert L.kache_amd_server_get_stats(None, None) == E
+    assert L.kache_amd_server_replay(None, None, None, 0, None) == E
+    assert L.kache_amd_clock() > 0
