@@ -46,7 +46,13 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--blocks-per-cu", type=int, default=0)
-    p.add_argument("--seed", type=int, default=0xC2)
+    p.add_argument("--seed", type=int, default=None)
+    p.add_argument("--workload", choices=["count-map", "serve"], default="count-map")
+    p.add_argument("--requests", type=int, default=2000, help="serve: requests per run")
+    p.add_argument("--serve-genome", type=int, default=2_000_000_000)
+    p.add_argument("--serve-expected", type=int, default=2_000_000_000)
+    p.add_argument("--cpu-requests", type=int, default=600)
+    p.add_argument("--cpu-genome", type=int, default=200_000_000)
     return p.parse_args()
 
 
@@ -124,6 +130,10 @@ def main():
     from kache_hash_amd import capi
 
     api = KA.api()
+    if a.workload == "serve":
+        return serve_main(a, KA, capi, api)
+    if a.seed is None:
+        a.seed = 0xC2
     nb = (a.reads + a.batch - 1) // a.batch
     steps = a.steps if a.steps is not None else max(nb - a.warmup, 1)
     need = a.warmup + steps
@@ -232,6 +242,233 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+# --- Serving workload ------------------------------------------------------
+
+SERVE_MIX = (0.35, 0.35, 0.05, 0.25)     # count, lookup, update (+1 if present), upsert (+1, new keys start at 1)
+_COMP = np.arange(256, dtype=np.uint8)
+for _a, _b in (b"AT", b"TA", b"CG", b"GC"):
+    _COMP[_a] = _b
+_CODE = np.zeros(256, np.uint8)
+for _i, _c in enumerate(b"ACGT"):
+    _CODE[_c] = _i
+_ACGT = np.frombuffer(b"ACGT", np.uint8)
+
+
+def serve_requests(capi, genome, n, seed, region):
+    """n requests: kind by SERVE_MIX; 1..10,000 sequences (log-uniform), of
+    lengths ~ lognormal(median 300, sigma 1) clamped to [1, 50,000], from
+    uniform starts in genome[:region], random strand, 1% substitutions.
+    Returns [(kind, batch)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        kind = int(rng.choice(4, p=SERVE_MIX))
+        m = min(int(np.exp(rng.uniform(0, np.log(10_001)))), 10_000)
+        lens = np.clip(np.rint(rng.lognormal(np.log(300), 1.0, m)), 1, 50_000).astype(np.int64)
+        starts = rng.integers(0, region - lens)
+        rc = np.repeat(rng.random(m) < 0.5, lens)
+        cum = np.zeros(m + 1, np.int64)
+        np.cumsum(lens, out=cum[1:])
+        j = np.arange(cum[-1]) - np.repeat(cum[:-1], lens)
+        L = np.repeat(lens, lens)
+        src = np.repeat(starts, lens) + np.where(rc, L - 1 - j, j)
+        b = genome[src]
+        b = np.where(rc, _COMP[b], b)
+        sub = np.flatnonzero(rng.random(len(b)) < 0.01)
+        b[sub] = _ACGT[(_CODE[b[sub]] + rng.integers(1, 4, len(sub))) % 4]
+        out.append((kind, capi.Batch.from_buffer(np.ascontiguousarray(b, np.uint8), cum[:-1], lens)))
+    return out
+
+
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def serve_run(capi, table, user, reqs, made, arrivals, wave_bytes, max_wave):
+    """One server over the table, fed `reqs` (as `made` requests) at
+    `arrivals`; summary dict."""
+    with capi.Server(table, user, wave_bytes, max_wave) as s:
+        res = s.replay(made, arrivals)
+        st = s.stats()
+    bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"served request failed: {bad[0]}")
+    lat = [r["done_s"] - r["submit_s"] for r in res]
+    span = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
+    pos = sum(b.positions() for _, b in reqs)
+    return {"kmers_per_s": pos / span, "requests_per_s": len(reqs) / span, "p50_ms": _pct(lat, 50) * 1e3,
+            "p99_ms": _pct(lat, 99) * 1e3, "mean_ms": float(np.mean(lat)) * 1e3, "waves": st["waves"],
+            "max_wave_requests": st["max_wave_requests"], "seconds": span}
+
+
+def cpu_serve(capi, reqs, genome, region, threads):
+    """The reference C ABI (oracle/_ref) serving requests from `threads`
+    workers under its threading contract: a request starts only when no
+    request of another class (reads, updates, upserts) runs.
+    Table: u32 map of genome[:region] (+1 per position), sized 1.3x.  Closed loop: all
+    requests queued at the start, run phased by class and in arrival order."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    path = O.ref_library()
+    if path is None:
+        return None
+    api = capi.Api(path)
+    t = capi.Table(api, 4, int(region * 1.3), 0.8, False, threads)    # headroom: no doubling while timed
+    users = [t.register() for _ in range(threads)]
+    # Prefill from all workers: the region in pieces of 1 Mbp (overlapping by k - 1).
+    piece = 1 << 20
+    starts = list(range(0, region, piece - 30))
+
+    def prefill(w):
+        n = C.c_uint64()
+        for i in range(w, len(starts), threads):
+            a0 = starts[i]
+            seg = np.ascontiguousarray(genome[a0:min(region, a0 + piece)])
+            ptr = np.array([seg.ctypes.data], np.uint64)
+            ln = np.array([len(seg)], np.uint64)
+            assert api.lib.kache_insert(t.h, users[w], ptr.ctypes.data, ln.ctypes.data, 1, capi.ADD, 1, C.byref(n)) == 0
+
+    th = [threading.Thread(target=prefill, args=(w,)) for w in range(threads)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    cls = [0 if k in (capi.REQ_COUNT, capi.REQ_LOOKUP) else (1 if k == capi.REQ_UPDATE else 2) for k, _ in reqs]
+
+    def run(order):
+        cv = threading.Condition()
+        state = {"next": 0, "cls": None, "inflight": 0}
+
+        def worker(w):
+            n = C.c_uint64()
+            while True:
+                with cv:
+                    while True:
+                        if state["next"] >= len(order):
+                            return
+                        i = order[state["next"]]
+                        if state["inflight"] == 0 or state["cls"] == cls[i]:
+                            break
+                        cv.wait()
+                    state["next"] += 1
+                    state["cls"] = cls[i]
+                    state["inflight"] += 1
+                k, b = reqs[i]
+                if k == capi.REQ_COUNT:
+                    out = np.zeros(max(b.n, 1), np.uint64)
+                    st = api.lib.kache_count(t.h, b.c_seqs, b.c_lens, b.n, out.ctypes.data)
+                elif k == capi.REQ_LOOKUP:
+                    p = max(b.positions(), 1)
+                    f, v, fw = np.zeros(p, np.uint8), np.zeros(4 * p, np.uint8), np.zeros(p, np.uint8)
+                    st = api.lib.kache_lookup(t.h, b.c_seqs, b.c_lens, b.n, f.ctypes.data, v.ctypes.data, fw.ctypes.data)
+                elif k == capi.REQ_UPDATE:
+                    st = api.lib.kache_update(t.h, b.c_seqs, b.c_lens, b.n, capi.ADD, 1, C.byref(n))
+                else:
+                    st = api.lib.kache_insert(t.h, users[w], b.c_seqs, b.c_lens, b.n, capi.ADD, 1, C.byref(n))
+                assert st == 0
+                with cv:
+                    state["inflight"] -= 1
+                    cv.notify_all()
+
+        th = [threading.Thread(target=worker, args=(w,)) for w in range(threads)]
+        t0 = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        return time.perf_counter() - t0
+
+    pos = sum(b.positions() for _, b in reqs)
+    # Arrival order: the plain calls, as concurrent as the contract allows.
+    dt_plain = run(list(range(len(reqs))))
+    # Phased: all reads, then all updates, then all upserts (the reordering a
+    # served form would make; the CPU path's best case).
+    dt = run(sorted(range(len(reqs)), key=lambda i: cls[i]))
+    t.close()
+    return {"value": pos / dt, "unit": "k-mer operations/s", "cores": threads, "kind": "reference",
+            "requests_per_s": len(reqs) / dt, "arrival_order_kmers_per_s": pos / dt_plain,
+            "sample": f"{len(reqs)} requests of the same mix and shapes from the first {region // 1_000_000} Mbp of the genome, "
+                      f"run closed-loop by {threads} threads through the reference C ABI, phased (reads, then updates, then "
+                      f"upserts; classes never overlap), against a resident u32 map of those {region // 1_000_000} Mbp "
+                      f"(far beyond the caches); arrival_order_kmers_per_s: the same requests in arrival order; "
+                      f"{os.path.basename(path)}",
+            "seconds": dt}
+
+
+def serve_main(a, KA, capi, api):
+    """Serving: a resident u32 count map of a 2 Gbp genome (sized for 2G keys
+    exactly); requests arrive as a Poisson process; served (waves) vs plain
+    per-request calls (one request per wave) at the same arrival rates."""
+    seed = 0xC5 if a.seed is None else a.seed
+    t_gen = time.perf_counter()
+    n = a.serve_genome
+    g = capi.DeviceBuffer(api, n + 1 + capi.TEXT_PAD)
+    api.check(api.lib.kache_amd_synth_genome(g.ptr, n, seed), "synth_genome")
+    api.check(api.lib.kache_amd_device_fill(g.ptr + n, ord("\n"), 1 + capi.TEXT_PAD), "fill")
+    t = KA.table(width=4, expected_keys=a.serve_expected)
+    u = t.register()
+    t.insert_device(g.ptr, n + 1, KA.ADD, 1, u)
+    genome = g.download(n)
+    g.free()
+    print(f"serve: genome {n} bp resident as {t.size()} keys in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    reqs = serve_requests(capi, genome, a.requests, seed + 1, n)
+    warm = serve_requests(capi, genome, max(a.requests // 10, 50), seed + 2, n)
+    pos = sum(b.positions() for _, b in reqs)
+    print(f"serve: {len(reqs)} requests, {pos} k-mer positions", file=sys.stderr, flush=True)
+    WAVE = 64 << 20
+    # Requests (and their output arrays) are made once and reused by every run.
+    made = [capi.make_request(4, k, b, capi.ADD, 1) for k, b in reqs]
+    wmade = [capi.make_request(4, k, b, capi.ADD, 1) for k, b in warm]
+    for m in made + wmade:
+        for arr in m[1].values():
+            arr.fill(1)         # fault the output pages in before any timing
+    serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1 << 20)       # warm-up: the first upserts double the table
+    serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1)
+    cap_served = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)
+    cap_plain = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1)
+    print(f"serve: capacity served {cap_served['requests_per_s']:.0f} req/s, plain {cap_plain['requests_per_s']:.0f} req/s",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed + 3)
+    sweep = []
+    for frac in (0.25, 0.5, 0.75, 0.9, 1.5, 2.5):
+        rate = frac * cap_plain["requests_per_s"]
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        row = {"rate_req_per_s": rate, "of_plain_capacity": frac,
+               "served": serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20)}
+        if frac <= 0.9:
+            row["plain"] = serve_run(capi, t, u, reqs, made, arr, WAVE, 1)
+        sweep.append(row)
+        print(f"serve: rate {rate:.0f}/s served p50 {row['served']['p50_ms']:.2f} p99 {row['served']['p99_ms']:.2f} ms"
+              + (f", plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms" if "plain" in row else ""),
+              file=sys.stderr, flush=True)
+    info = t.info()
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        creqs = serve_requests(capi, genome, a.cpu_requests, seed + 4, a.cpu_genome)
+        print(f"serve: reference CPU path, {len(creqs)} requests, {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_serve(capi, creqs, genome, a.cpu_genome, threads)
+    out = {
+        "metric": "k-mer upserts/s + lookups/s (k=31 count map); p50/p99 served-batch latency",
+        "value": cap_served["kmers_per_s"], "unit": "k-mer operations/s (served, closed loop)", "n_gpus": 1,
+        "steps": len(reqs), "warmup": len(warm), "ms_per_step": cap_served["seconds"] * 1e3 / len(reqs),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded genome on device, requests on host)",
+        "config": {"workload": "serving: resident <31,u32,22> map of a 2 Gbp genome (sized for 2G keys); requests "
+                               "35% count / 35% lookup / 5% update +1 / 25% upsert +1, 1-10,000 sequences "
+                               "(log-uniform), lengths lognormal median 300 (max 50k), 1% substitutions; Poisson arrivals",
+                   "requests": len(reqs), "kmer_positions": pos, "wave_bytes": WAVE, "genome_bp": n},
+        "p50_ms": cap_served["p50_ms"], "p99_ms": cap_served["p99_ms"],
+        "capacity": {"served": cap_served, "plain_per_request": cap_plain},
+        "sweep": sweep,
+        "table": {"keys": t.size(), "buckets": info["buckets"], "resizes": info["resizes"], "device_bytes": info["device_bytes"]},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -40,6 +40,15 @@ int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_byte
 int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, const uint64_t* bounds, uint64_t n_batches, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
 int kache_amd_count_batches(const kache_table* t, const char* text, const uint64_t* bounds, uint64_t n_batches, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;
 
+/* kache_lookup over all n sequences in one pass, for groups of consecutive
+ * sequences with outputs of their own: group g holds sequences
+ * [group_first[g], group_first[g + 1]) (the last one up to n; group_first[0]
+ * is 0, non-decreasing), and its outputs go to found[g], values[g], fwd[g]
+ * laid out as kache_lookup of those sequences alone.  `values` / `fwd` (the
+ * arrays) or any of their entries may be NULL. */
+int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, const uint64_t* group_first,
+                            size_t n_groups, uint8_t* const* found, void* const* values, uint8_t* const* fwd) KACHE_NOEXCEPT;
+
 /* The number of k-mer positions of a device text present in the table
  * (`hits`) and, unless NULL, of all its k-mer positions (`positions`). */
 int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text_bytes, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;
@@ -82,6 +91,9 @@ int kache_amd_device_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
 int kache_amd_device_free(void* ptr) KACHE_NOEXCEPT;
 int kache_amd_copy(void* dst, const void* src, uint64_t bytes, int to_device) KACHE_NOEXCEPT;
 int kache_amd_device_fill(void* ptr, uint8_t value, uint64_t bytes) KACHE_NOEXCEPT;
+/* Pinned host memory (hipHostMalloc): copies from and to it run at full link speed. */
+int kache_amd_host_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
+int kache_amd_host_free(void* ptr) KACHE_NOEXCEPT;
 
 /* Seeded synthetic workloads, made on the device from a counter-based hash:
  * a genome of n uniform ACGT bases; and n_reads reads of read_len bases from
@@ -104,7 +116,8 @@ const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
  * (COUNT and LOOKUP) merged into one call each, then its updates one by one,
  * then its upserts, one pipelined batch call (kache_amd_insert_batches) per
  * run of equal op and operand, a batch per request.  Requests arriving while
- * a wave runs join the next one.  Each request's results are those of
+ * a wave runs join the next one; a request completes when its phase ends.
+ * Each request's results are those of
  * running all requests alone, one after another, in served order: by wave,
  * then phase (reads 0, updates 1, upserts 2), then ticket.  With
  * max_wave_requests = 1 every request is one plain C-ABI call.

@@ -94,6 +94,9 @@ AMD_SYMBOLS = {
     "kache_amd_device_free": (C.c_int, [_P]),
     "kache_amd_copy": (C.c_int, [_P, _P, C.c_uint64, C.c_int]),
     "kache_amd_device_fill": (C.c_int, [_P, C.c_uint8, C.c_uint64]),
+    "kache_amd_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
+    "kache_amd_host_free": (C.c_int, [_P]),
+    "kache_amd_lookup_groups": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P, C.c_size_t, _P, _P, _P]),
     "kache_amd_synth_genome": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
     "kache_amd_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "kache_amd_host_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
@@ -401,6 +404,31 @@ class DeviceBuffer:
         return out.view(dtype)
 
 
+def make_request(width, kind, seqs, op=KEEP, x=0, values=True, fwd=True):
+    """A serving request over a table of `width`-byte values and the arrays it
+    writes: (Request, outputs, batch); reusable across servers and runs."""
+    b = as_batch(seqs)
+    r = Request(kind=kind, op=op, x=x, seqs=b.c_seqs, lens=b.c_lens, n=b.n)
+    out = {}
+    if kind == REQ_COUNT:
+        out["counts"] = np.zeros(max(b.n, 1), np.uint64)
+        r.counts = out["counts"].ctypes.data
+    elif kind == REQ_LOOKUP:
+        p = max(b.positions(), 1)
+        out["found"] = np.zeros(p, np.uint8)
+        r.found = out["found"].ctypes.data
+        if values and width:
+            out["values"] = np.zeros(p * width, np.uint8)
+            r.values = out["values"].ctypes.data
+        if fwd:
+            out["fwd"] = np.zeros(p, np.uint8)
+            r.fwd = out["fwd"].ctypes.data
+    else:
+        out["counts"] = np.zeros(1, np.uint64)
+        r.counts = out["counts"].ctypes.data
+    return r, out, b
+
+
 class Server:
     """The serving loop of a table (kache_amd_server_*): requests submitted
     from any thread run in waves on one worker; `user` runs the upserts."""
@@ -430,27 +458,8 @@ class Server:
         self.close()
 
     def request(self, kind, seqs, op=KEEP, x=0, values=True, fwd=True):
-        """A request and the arrays it writes: (Request, outputs, keep-alive)."""
-        b = as_batch(seqs)
-        r = Request(kind=kind, op=op, x=x, seqs=b.c_seqs, lens=b.c_lens, n=b.n)
-        out = {}
-        if kind == REQ_COUNT:
-            out["counts"] = np.zeros(max(b.n, 1), np.uint64)
-            r.counts = out["counts"].ctypes.data
-        elif kind == REQ_LOOKUP:
-            p = max(b.positions(), 1)
-            out["found"] = np.zeros(p, np.uint8)
-            r.found = out["found"].ctypes.data
-            if values and self.table.width:
-                out["values"] = np.zeros(p * self.table.width, np.uint8)
-                r.values = out["values"].ctypes.data
-            if fwd:
-                out["fwd"] = np.zeros(p, np.uint8)
-                r.fwd = out["fwd"].ctypes.data
-        else:
-            out["counts"] = np.zeros(1, np.uint64)
-            r.counts = out["counts"].ctypes.data
-        return r, out, b
+        """A request and the arrays it writes: (Request, outputs, batch)."""
+        return make_request(self.table.width, kind, seqs, op, x, values, fwd)
 
     def submit(self, req):
         """Submits (Request, outputs, batch) from request(); returns the ticket."""

@@ -86,10 +86,10 @@ struct kache_amd_server
     bool stopping = false;
     std::thread worker;
 
-    // Staging of a wave's upserts: host bytes, their device copy.
-    std::vector<char> h_text;
+    // Staging of a wave's upserts: pinned host bytes, their device copy.
+    char* h_text = nullptr;
     void* d_text = nullptr;
-    uint64_t d_bytes = 0;
+    uint64_t h_bytes = 0, d_bytes = 0;
 
     kache_amd_server_stats stats{};
 
@@ -109,6 +109,8 @@ struct kache_amd_server
         worker.join();
         if(d_text)
             kache_amd_device_free(d_text);
+        if(h_text)
+            kache_amd_host_free(h_text);
     }
 
     void loop()
@@ -139,14 +141,22 @@ struct kache_amd_server
             for(auto& q : wave)
                 q->t_start = t0, q->wave = wave_id, q->phase = phase_of(q->r.kind);
             run(wave);
-            const double t1 = now_s();
-            {
-                std::lock_guard<std::mutex> lock(mu);
-                for(auto& q : wave)
-                    q->t_done = t1, q->done = true;
-            }
-            cv_done.notify_all();
         }
+    }
+
+    // Marks requests done as soon as their phase ends: later phases of the
+    // wave cannot change their results.
+    void finish(const std::vector<Req*>& rs)
+    {
+        if(rs.empty())
+            return;
+        const double t1 = now_s();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            for(Req* q : rs)
+                q->t_done = t1, q->done = true;
+        }
+        cv_done.notify_all();
     }
 
     void run(std::vector<std::shared_ptr<Req>>& wave)
@@ -161,9 +171,12 @@ struct kache_amd_server
             default: inserts.push_back(q.get()); break;
             }
         run_counts(counts);
+        finish(counts);
         run_lookups(lookups);
+        finish(lookups);
         for(Req* q : updates)
             q->status = kache_update(t, q->r.seqs, q->r.lens, q->r.n, q->r.op, q->r.x, q->r.counts);
+        finish(updates);
         for(size_t a = 0; a < inserts.size();)
         {
             size_t b = a + 1;
@@ -172,6 +185,7 @@ struct kache_amd_server
             run_inserts(inserts, a, b);
             a = b;
         }
+        finish(inserts);
     }
 
     // All sequences of the requests, in order, as one call's arrays.
@@ -221,30 +235,19 @@ struct kache_amd_server
         std::vector<const char*> seqs;
         std::vector<size_t> lens;
         merge(rs, seqs, lens);
-        uint64_t total = 0;
-        bool want_values = false, want_fwd = false;
-        for(Req* q : rs)
-            total += positions_of(q->r), want_values |= q->r.values != nullptr, want_fwd |= q->r.fwd != nullptr;
-        std::vector<uint8_t> found(std::max<uint64_t>(total, 1));
-        std::vector<uint8_t> values(want_values && width ? std::max<uint64_t>(total, 1) * width : 0);
-        std::vector<uint8_t> fwd(want_fwd ? std::max<uint64_t>(total, 1) : 0);
-        const int st = kache_lookup(t, seqs.data(), lens.data(), seqs.size(), found.data(), values.empty() ? nullptr : values.data(),
-                                    fwd.empty() ? nullptr : fwd.data());
+        std::vector<uint64_t> first;
+        std::vector<uint8_t*> found, fwd;
+        std::vector<void*> values;
         uint64_t o = 0;
         for(Req* q : rs)
         {
-            const uint64_t p = positions_of(q->r);
-            q->status = st;
-            if(st == KACHE_OK && p)
-            {
-                std::memcpy(q->r.found, found.data() + o, p);
-                if(q->r.values && width)
-                    std::memcpy(q->r.values, values.data() + o * width, p * width);
-                if(q->r.fwd)
-                    std::memcpy(q->r.fwd, fwd.data() + o, p);
-            }
-            o += p;
+            first.push_back(o), o += q->r.n;
+            found.push_back(q->r.found), values.push_back(q->r.values), fwd.push_back(q->r.fwd);
         }
+        const int st = kache_amd_lookup_groups(t, seqs.data(), lens.data(), seqs.size(), first.data(), rs.size(), found.data(),
+                                               values.data(), fwd.data());
+        for(Req* q : rs)
+            q->status = st;
     }
 
     void run_inserts(const std::vector<Req*>& ins, size_t a, size_t b)
@@ -259,34 +262,46 @@ struct kache_amd_server
         uint64_t bytes = 0;
         for(size_t i = a; i < b; ++i)
             bytes += ins[i]->bytes;
-        h_text.resize(bytes + KACHE_AMD_TEXT_PAD);
-        uint64_t o = 0;
-        for(size_t i = a; i < b; ++i)
-        {
-            const auto& r = ins[i]->r;
-            for(size_t j = 0; j < r.n; ++j)
-            {
-                if(r.lens[j])
-                    std::memcpy(h_text.data() + o, r.seqs[j], r.lens[j]);
-                o += r.lens[j];
-                h_text[o++] = '\n';
-            }
-            bounds.push_back(o);
-        }
-        std::memset(h_text.data() + o, '\n', KACHE_AMD_TEXT_PAD);
+        const uint64_t need = bytes + KACHE_AMD_TEXT_PAD;
+        const uint64_t want = std::max<uint64_t>(bytes, wave_bytes) + KACHE_AMD_TEXT_PAD;
         int st = KACHE_OK;
-        if(bytes + KACHE_AMD_TEXT_PAD > d_bytes)
+        if(need > h_bytes)
+        {
+            if(h_text)
+                kache_amd_host_free(h_text), h_text = nullptr, h_bytes = 0;
+            void* p = nullptr;
+            st = kache_amd_host_alloc(want, &p);
+            if(st == KACHE_OK)
+                h_text = static_cast<char*>(p), h_bytes = want;
+        }
+        if(st == KACHE_OK && need > d_bytes)
         {
             if(d_text)
                 kache_amd_device_free(d_text), d_text = nullptr, d_bytes = 0;
-            const uint64_t want = std::max<uint64_t>(bytes, wave_bytes) + KACHE_AMD_TEXT_PAD;
             st = kache_amd_device_alloc(want, &d_text);
             if(st == KACHE_OK)
                 d_bytes = want;
         }
+        if(st == KACHE_OK)
+        {
+            uint64_t o = 0;
+            for(size_t i = a; i < b; ++i)
+            {
+                const auto& r = ins[i]->r;
+                for(size_t j = 0; j < r.n; ++j)
+                {
+                    if(r.lens[j])
+                        std::memcpy(h_text + o, r.seqs[j], r.lens[j]);
+                    o += r.lens[j];
+                    h_text[o++] = '\n';
+                }
+                bounds.push_back(o);
+            }
+            std::memset(h_text + o, '\n', KACHE_AMD_TEXT_PAD);
+        }
         std::vector<uint64_t> out(b - a);
         if(st == KACHE_OK)
-            st = kache_amd_copy(d_text, h_text.data(), bytes + KACHE_AMD_TEXT_PAD, 1);
+            st = kache_amd_copy(d_text, h_text, bytes + KACHE_AMD_TEXT_PAD, 1);
         if(st == KACHE_OK)
             st = kache_amd_insert_batches(t, user, static_cast<const char*>(d_text), bounds.data(), b - a, ins[a]->r.op, ins[a]->r.x, out.data());
         for(size_t i = a; i < b; ++i)

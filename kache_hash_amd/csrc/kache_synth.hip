@@ -121,6 +121,21 @@ int kache_amd_device_free(void* ptr) noexcept
     return e == hipSuccess ? KACHE_OK : fail(e);
 }
 
+int kache_amd_host_alloc(uint64_t bytes, void** ptr) noexcept
+{
+    if(ptr == nullptr)
+        return KACHE_EINVAL;
+    *ptr = nullptr;
+    const hipError_t e = hipHostMalloc(ptr, std::max<uint64_t>(bytes, 1), hipHostMallocDefault);
+    return e == hipSuccess ? KACHE_OK : fail(e);
+}
+
+int kache_amd_host_free(void* ptr) noexcept
+{
+    const hipError_t e = hipHostFree(ptr);
+    return e == hipSuccess ? KACHE_OK : fail(e);
+}
+
 int kache_amd_copy(void* dst, const void* src, uint64_t bytes, int to_device) noexcept
 {
     if(bytes == 0)

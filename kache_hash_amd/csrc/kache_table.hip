@@ -947,8 +947,26 @@ struct kache_table
         }
     }
 
+    // Where a group of consecutive sequences [first, next group's first)
+    // puts its lookup outputs (values and fwd may be null).
+    struct LookupDest
+    {
+        uint64_t first;
+        uint8_t* found;
+        void* values;
+        uint8_t* fwd;
+    };
+
     // capi/Kmer_Table.hpp:163-190.
     void lookup(const char* const* seqs, const size_t* lens, size_t n, uint8_t* found, void* values, uint8_t* fwd) const
+    {
+        const LookupDest d{0, found, values, fwd};
+        lookup_groups(seqs, lens, n, &d, 1);
+    }
+
+    // One lookup pass over all sequences; each group's outputs, laid out as
+    // a lookup of its own sequences alone, are copied to its destinations.
+    void lookup_groups(const char* const* seqs, const size_t* lens, size_t n, const LookupDest* g, size_t ng) const
     {
         std::lock_guard<std::mutex> lock(mu);
         std::vector<uint64_t> o(n + 1);
@@ -958,6 +976,9 @@ struct kache_table
         o[n] = p;
         if(p == 0)
             return;
+        bool values = false, fwd = false;
+        for(size_t i = 0; i < ng; ++i)
+            values |= g[i].values != nullptr, fwd |= g[i].fwd != nullptr;
 
         const uint32_t vb = value_bytes;
         d_out.reserve(p);                       // found
@@ -987,11 +1008,17 @@ struct kache_table
             launch_query<Q_LOOKUP>(text, st.text_len, a);
             sync();
         }
-        HIP(hipMemcpyAsync(found, d_out.p, p, hipMemcpyDeviceToHost, stream));
-        if(values)
-            HIP(hipMemcpyAsync(values, d_out2.p, p * vb, hipMemcpyDeviceToHost, stream));
-        if(fwd)
-            HIP(hipMemcpyAsync(fwd, d_out3.p, p, hipMemcpyDeviceToHost, stream));
+        for(size_t i = 0; i < ng; ++i)
+        {
+            const uint64_t b = o[g[i].first], e = o[i + 1 < ng ? g[i + 1].first : n];
+            if(e == b)
+                continue;
+            HIP(hipMemcpyAsync(g[i].found, d_out.as<uint8_t>() + b, e - b, hipMemcpyDeviceToHost, stream));
+            if(g[i].values)
+                HIP(hipMemcpyAsync(g[i].values, d_out2.as<uint8_t>() + b * vb, (e - b) * vb, hipMemcpyDeviceToHost, stream));
+            if(g[i].fwd)
+                HIP(hipMemcpyAsync(g[i].fwd, d_out3.as<uint8_t>() + b, e - b, hipMemcpyDeviceToHost, stream));
+        }
         sync();
     }
 
@@ -1298,6 +1325,29 @@ int kache_lookup(const kache_table* t, const char* const* seqs, const size_t* le
     if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && found == nullptr) || (t->value_bytes == 0 && values != nullptr))
         return KACHE_EINVAL;
     return guard([&](){ t->lookup(seqs, lens, n, found, values, fwd); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, const uint64_t* group_first,
+                            size_t n_groups, uint8_t* const* found, void* const* values, uint8_t* const* fwd) noexcept
+{
+    if(t == nullptr || !valid(seqs, lens, n) || (n_groups > 0 && (group_first == nullptr || found == nullptr)))
+        return KACHE_EINVAL;
+    for(size_t i = 0; i < n_groups; ++i)
+        if(group_first[i] > n || (i > 0 && group_first[i] < group_first[i - 1]) || found[i] == nullptr ||
+           (values && values[i] && t->value_bytes == 0))
+            return KACHE_EINVAL;
+    if(n_groups > 0 && group_first[0] != 0)
+        return KACHE_EINVAL;
+    if(n_groups == 0)
+        return n == 0 ? static_cast<int>(KACHE_OK) : static_cast<int>(KACHE_EINVAL);
+    return guard([&]()
+    {
+        std::vector<kache_table::LookupDest> d(n_groups);
+        for(size_t i = 0; i < n_groups; ++i)
+            d[i] = {group_first[i], found[i], values ? values[i] : nullptr, fwd ? fwd[i] : nullptr};
+        t->lookup_groups(seqs, lens, n, d.data(), n_groups);
+        return static_cast<int>(KACHE_OK);
+    });
 }
 
 int kache_get(const kache_table* t, const char* kmer, uint8_t* found, uint64_t* value) noexcept

@@ -220,3 +220,39 @@ def test_replay_open_loop(api):
     assert (sub >= arrivals - 1e-4).all() and (sub <= arrivals + 0.05).all()
     assert all(r["submit_s"] <= r["start_s"] <= r["done_s"] for r in res)
     assert sorted(r["ticket"] for r in res) == [r["ticket"] for r in res]
+
+
+def test_lookup_groups_equals_separate_lookups(api):
+    """kache_amd_lookup_groups: one pass, each group's outputs laid out as a
+    lookup of its own sequences (empty groups and groups without k-mers
+    included), with and without values / fwd."""
+    import ctypes as C
+    g = G.genome(100_000, 57)
+    rng = np.random.default_rng(57)
+    t = capi.Table(api, 2, expected_keys=200_000)
+    u = t.register()
+    t.insert(random_seqs(rng, g, 300), capi.ADD, 1, u)
+    groups = [random_seqs(rng, g, int(m)) for m in (5, 0, 17, 1, 0, 40, 3)]
+    groups[3] = [b"ACGTN" * 4]          # a group without k-mers
+    seqs = [s for grp in groups for s in grp]
+    b = capi.Batch(seqs)
+    first = np.cumsum([0] + [len(x) for x in groups[:-1]]).astype(np.uint64)
+    for with_values, with_fwd in ((True, True), (False, True), (True, False)):
+        outs = [capi.make_request(2, capi.REQ_LOOKUP, grp, values=with_values, fwd=with_fwd)[1] for grp in groups]
+        P = C.c_void_p * len(groups)
+        found = P(*[o["found"].ctypes.data for o in outs])
+        values = P(*[o["values"].ctypes.data if "values" in o else None for o in outs])
+        fwd = P(*[o["fwd"].ctypes.data if "fwd" in o else None for o in outs])
+        api.check(api.lib.kache_amd_lookup_groups(t.h, b.c_seqs, b.c_lens, b.n, first.ctypes.data, len(groups), found,
+                                                  values if with_values else None, fwd if with_fwd else None), "lookup_groups")
+        for grp, o in zip(groups, outs):
+            f, v, fw = t.lookup(grp)
+            p = len(f)
+            assert np.array_equal(o["found"][:p], f)
+            if with_values:
+                assert np.array_equal(o["values"][:2 * p].view(np.uint16).astype(np.uint64), v)
+            if with_fwd:
+                assert np.array_equal(o["fwd"][:p], fw)
+    bad = np.array([1, 0], np.uint64)
+    one = (C.c_void_p * 2)(1, 1)
+    assert api.lib.kache_amd_lookup_groups(t.h, b.c_seqs, b.c_lens, b.n, bad.ctypes.data, 2, one, None, None) == capi.KACHE_EINVAL
