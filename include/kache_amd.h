@@ -55,8 +55,25 @@ int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text
 
 /* The C++ API's wrapping `v + x` (a map's `upsert(..., [](v){ return v + x; }, x)`,
  * include/Streaming_Kmer_Hash_Table.hpp:672), which the C ABI's saturating
- * KACHE_ADD does not cover: op code for the calls above and kache_insert. */
+ * KACHE_ADD does not cover: op code for the device-text and host-text calls
+ * of this header.  The C ABI's own entry points refuse it (EINVAL), as the
+ * reference's do (test/capi.cpp:455). */
 #define KACHE_AMD_ADD_WRAP 6
+
+/* kache_insert / kache_update over a host text: sequences separated by
+ * non-ACGT bytes (e.g. newline-separated reads), in pageable host memory.
+ * The text streams through two pinned buffers in chunks of 16 MiB: while one
+ * chunk's kernels run, the next is packed and copied on a second stream.
+ * `op` may also be KACHE_AMD_ADD_WRAP.  Validation as kache_insert /
+ * kache_update (an update of a set is EINVAL). */
+int kache_amd_insert_text(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_update_text(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
+
+/* Histogram of a map's values, as the reference's counter computes it
+ * (counter/Kmer_Set_Counter.cpp:97-120): out[c] = the number of keys with
+ * value c for c in [1, max], out[0] = 0; `out` holds max + 1 entries.  One
+ * device pass over the table (EINVAL on a set or for max > 2^30). */
+int kache_amd_histogram(const kache_table* t, uint64_t max, uint64_t* out) KACHE_NOEXCEPT;
 
 /* Device describer, for tests: for each position p of [0, text_bytes) puts
  * whether a 31-mer starts there (valid), its canonical key, minimizer hash,

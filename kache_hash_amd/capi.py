@@ -97,6 +97,9 @@ AMD_SYMBOLS = {
     "kache_amd_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
     "kache_amd_host_free": (C.c_int, [_P]),
     "kache_amd_lookup_groups": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P, C.c_size_t, _P, _P, _P]),
+    "kache_amd_insert_text": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_update_text": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_histogram": (C.c_int, [_P, C.c_uint64, _P]),
     "kache_amd_synth_genome": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
     "kache_amd_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "kache_amd_host_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
@@ -359,6 +362,26 @@ class Table:
         n, p = C.c_uint64(), C.c_uint64()
         self.api.check(self.api.lib.kache_amd_count_device(self.h, text_ptr, text_bytes, C.byref(n), C.byref(p)), "kache_amd_count_device")
         return (n.value, p.value) if positions else n.value
+
+    def insert_text(self, text, op=KEEP, x=0, user=0):
+        """Inserts a host text (sequences separated by non-ACGT bytes): new keys."""
+        buf = np.frombuffer(text, np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text, np.uint8)
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_insert_text(self.h, user, buf.ctypes.data if buf.size else None, buf.size, op, x, C.byref(n)), "kache_amd_insert_text")
+        return n.value
+
+    def update_text(self, text, op, x):
+        """Updates the present k-mers of a host text: present positions."""
+        buf = np.frombuffer(text, np.uint8) if isinstance(text, (bytes, bytearray)) else np.ascontiguousarray(text, np.uint8)
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_update_text(self.h, buf.ctypes.data if buf.size else None, buf.size, op, x, C.byref(n)), "kache_amd_update_text")
+        return n.value
+
+    def histogram(self, max_count):
+        """out[c] = keys with value c, c in [1, max_count]; out[0] = 0."""
+        out = np.zeros(max_count + 1, np.uint64)
+        self.api.check(self.api.lib.kache_amd_histogram(self.h, max_count, out.ctypes.data), "kache_amd_histogram")
+        return out
 
     def profile(self, on=True):
         self.api.check(self.api.lib.kache_amd_profile(self.h, int(on)), "kache_amd_profile")

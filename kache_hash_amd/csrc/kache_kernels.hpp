@@ -432,6 +432,55 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
     block_add(acc, out, red);
 }
 
+// Histogram of a map's values: out[v] += number of keys with value v, for v
+// in [1, max] (counter/Kmer_Set_Counter.cpp:97-120).  LDS bins per block
+// when they fit (dynamic shared memory of (max + 1) words), else global
+// atomics.
+template <bool LDS>
+__global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, uint64_t* out)
+{
+    extern __shared__ uint32_t bins[];
+    if constexpr(LDS)
+    {
+        for(uint64_t i = threadIdx.x; i <= max; i += blockDim.x)
+            bins[i] = 0;
+        __syncthreads();
+    }
+    const uint64_t main_n = T.cap * B, all = main_n + T.ov_mask + 1;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
+    {
+        const uint64_t* s;
+        if(g < main_n)
+        {
+            uint8_t* bk = bucket_at<true>(T, g / B);
+            if(!bk[g % B])
+                continue;
+            s = slot_at<true>(bk, g % B);
+        }
+        else
+        {
+            s = T.ov + (g - main_n) * Geo<true>::slot_words;
+            if(!s[0])
+                continue;
+        }
+        const uint64_t v = s[1];
+        if(v < 1 || v > max)
+            continue;
+        if constexpr(LDS)
+            atomicAdd(&bins[v], 1u);
+        else
+            add64(out + v, 1);
+    }
+    if constexpr(LDS)
+    {
+        __syncthreads();
+        for(uint64_t i = threadIdx.x; i <= max; i += blockDim.x)
+            if(bins[i])
+                add64(out + i, bins[i]);
+    }
+}
+
 // Keys of the main table at their primary bucket, and elsewhere
 // (include/Streaming_Kmer_Hash_Table.hpp:1109-1120, 2728-2752).
 template <bool MAP>

@@ -82,6 +82,7 @@ constexpr uint64_t MAX_CAP = (uint64_t(1) << 32) - 1;   // include/Streaming_Kme
 constexpr uint64_t OV_MIN = 1024;                       // :67
 constexpr uint64_t LAUNCH_POS = uint64_t(1) << 24;      // positions per insert launch
 constexpr uint64_t STAGE_BYTES = uint64_t(1) << 28;     // host sequence bytes staged at once
+constexpr uint64_t PIPE_BYTES = uint64_t(1) << 24;      // chunk of the double-buffered host pipeline
 constexpr uint64_t COLLECT_BUCKETS = uint64_t(1) << 18; // buckets per iteration copy-out
 
 uint64_t ceil_pow2(uint64_t x)
@@ -225,6 +226,12 @@ struct kache_table
 
     // Staging and scratch.
     mutable HostBuf h_text, h_misc;
+    // Double-buffered staging of host sequences: chunk i + 1 is packed and
+    // copied (on copy_stream) while chunk i's kernels run.
+    mutable HostBuf h_pipe[2];
+    mutable DevBuf d_pipe[2];
+    mutable hipStream_t copy_stream = nullptr;
+    mutable hipEvent_t ev_copied[2] = {}, ev_freed[2] = {};
     mutable DevBuf d_text, d_misc, d_out, d_out2, d_out3;
     mutable Counters* d_cnt = nullptr;
     mutable Counters* h_cnt = nullptr;
@@ -241,6 +248,12 @@ struct kache_table
         : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0)
     {
         HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+        for(int i = 0; i < 2; ++i)
+        {
+            HIP(hipEventCreateWithFlags(&ev_copied[i], hipEventDisableTiming));
+            HIP(hipEventCreateWithFlags(&ev_freed[i], hipEventDisableTiming));
+        }
 
         d_cnt = dev_alloc<Counters>(1);
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(Counters), hipHostMallocDefault));
@@ -286,6 +299,15 @@ struct kache_table
             if(e)
                 (void)hipEventDestroy(e), e = nullptr;
         ev_pool.clear();
+        if(copy_stream)
+            (void)hipStreamSynchronize(copy_stream), (void)hipStreamDestroy(copy_stream), copy_stream = nullptr;
+        for(int i = 0; i < 2; ++i)
+        {
+            if(ev_copied[i])
+                (void)hipEventDestroy(ev_copied[i]), ev_copied[i] = nullptr;
+            if(ev_freed[i])
+                (void)hipEventDestroy(ev_freed[i]), ev_freed[i] = nullptr;
+        }
         if(stream)
             (void)hipStreamDestroy(stream), stream = nullptr;
     }
@@ -540,6 +562,11 @@ struct kache_table
     // into pieces that overlap by k - 1 characters.  Returns false when done.
     bool stage_next(const char* const* seqs, const size_t* lens, size_t n, SeqCursor& cur, Stage& st) const
     {
+        return stage_into(seqs, lens, n, cur, st, h_text, STAGE_BYTES);
+    }
+
+    bool stage_into(const char* const* seqs, const size_t* lens, size_t n, SeqCursor& cur, Stage& st, HostBuf& hb, uint64_t limit) const
+    {
         st.text_len = 0, st.seg_start.clear(), st.seg_q.clear(), st.seg_off.clear();
         while(cur.q < n && (lens[cur.q] < static_cast<size_t>(K) || cur.off + K > lens[cur.q]))
             cur.q++, cur.off = 0;
@@ -548,11 +575,11 @@ struct kache_table
 
         // Size the stage.
         uint64_t need = 0;
-        for(size_t q = cur.q; q < n && need < STAGE_BYTES; ++q)
+        for(size_t q = cur.q; q < n && need < limit; ++q)
             need += (q == cur.q ? lens[q] - cur.off : lens[q]) + 1;
-        need = std::min<uint64_t>(need, STAGE_BYTES + K);
-        h_text.reserve(need + TEXT_PAD);
-        char* out = h_text.as<char>();
+        need = std::min<uint64_t>(need, limit + K);
+        hb.reserve(need + TEXT_PAD);
+        char* out = hb.as<char>();
 
         while(cur.q < n)
         {
@@ -656,17 +683,125 @@ struct kache_table
         return inserted;
     }
 
-    uint64_t insert_host(uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, bool oriented, uint64_t x, uint64_t xr)
+    // Runs `work(text, text_len)` on each chunk of the caller's sequences, as
+    // packed by stage_into in chunks of about PIPE_BYTES: while chunk i's
+    // kernels run on `stream`, chunk i + 1 is packed into the other pinned
+    // buffer and copied on copy_stream.  `work` enqueues its kernels on
+    // `stream` and may synchronize it; chunk i + 1's copy never waits for it.
+    template <typename Work>
+    void host_pipeline(const char* const* seqs, const size_t* lens, size_t n, const Work& work) const
+    {
+        SeqCursor cur;
+        pipeline([&](Stage& st, HostBuf& hb){ return stage_into(seqs, lens, n, cur, st, hb, PIPE_BYTES); }, work);
+    }
+
+    // The same over a host text: sequences separated by non-ACGT bytes, cut
+    // into chunks of about PIPE_BYTES that end after such a byte (a run of
+    // bases longer than a chunk is cut with k - 1 bases of overlap).
+    template <typename Work>
+    void text_pipeline(const char* text, uint64_t bytes, const Work& work) const
+    {
+        uint64_t off = 0;
+        pipeline([&](Stage& st, HostBuf& hb)
+        {
+            st.text_len = 0;
+            if(off >= bytes)
+                return false;
+            uint64_t end = std::min(bytes, off + PIPE_BYTES), next = end;
+            if(end < bytes)
+            {
+                uint64_t e = end;
+                while(e > off && is_base(static_cast<uint8_t>(text[e - 1])))
+                    e--;
+                if(e > off)
+                    end = next = e;
+                else
+                    next = end - (K - 1);
+            }
+            hb.reserve(end - off + 1 + TEXT_PAD);
+            char* out = hb.as<char>();
+            std::memcpy(out, text + off, end - off);
+            out[end - off] = '\n';
+            std::memset(out + end - off + 1, '\n', TEXT_PAD);
+            st.text_len = end - off + 1;
+            off = next;
+            return true;
+        }, work);
+    }
+
+    template <typename Fill, typename Work>
+    void pipeline(const Fill& fill, const Work& work) const
+    {
+        Stage st[2];
+        const auto stage = [&](int slot)
+        {
+            // The pinned buffer is free once its last copy ended; the device
+            // text once the kernels of two chunks ago are done.
+            HIP(hipEventSynchronize(ev_copied[slot]));
+            if(!fill(st[slot], h_pipe[slot]))
+                return false;
+            const uint64_t bytes = st[slot].text_len + TEXT_PAD;
+            if(bytes > d_pipe[slot].bytes)
+            {
+                HIP(hipStreamSynchronize(stream));
+                d_pipe[slot].reserve(bytes);
+            }
+            HIP(hipStreamWaitEvent(copy_stream, ev_freed[slot], 0));
+            HIP(hipMemcpyAsync(d_pipe[slot].p, h_pipe[slot].p, bytes, hipMemcpyHostToDevice, copy_stream));
+            HIP(hipEventRecord(ev_copied[slot], copy_stream));
+            return true;
+        };
+        try
+        {
+            bool have = stage(0);
+            for(int slot = 0; have; slot ^= 1)
+            {
+                HIP(hipStreamWaitEvent(stream, ev_copied[slot], 0));
+                work(d_pipe[slot].as<uint8_t>(), st[slot].text_len, [&]{ return stage(slot ^ 1); }, have);
+                HIP(hipEventRecord(ev_freed[slot], stream));
+            }
+        }
+        catch(...)
+        {
+            (void)hipStreamSynchronize(copy_stream);
+            throw;
+        }
+    }
+
+    // Inserts host sequences (seqs, lens, n) or, with seqs null, a host text.
+    uint64_t insert_host(uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, bool oriented, uint64_t x, uint64_t xr,
+                         const char* host_text = nullptr, uint64_t text_bytes = 0)
     {
         std::lock_guard<std::mutex> lock(mu);
         CallSched s = sched_begin(user);
         uint64_t inserted = 0;
-        SeqCursor cur;
-        Stage st;
         try
         {
-            while(stage_next(seqs, lens, n, cur, st))
-                inserted += insert_text(s, op, oriented, upload(st), st.text_len, x, xr);
+            const auto work = [&](const uint8_t* text, uint64_t text_len, const auto& stage_next_chunk, bool& more)
+            {
+                // Launches of at most LAUNCH_POS positions; the next chunk is
+                // staged while the first one runs, before its count is read.
+                bool staged = false;
+                for(uint64_t b = 0; b < text_len; b += LAUNCH_POS)
+                {
+                    const uint64_t e = std::min(text_len, b + LAUNCH_POS);
+                    ov_room(e - b);
+                    timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
+                    if(!staged)
+                        more = stage_next_chunk(), staged = true;
+                    const Counters c = read_counters();
+                    if(c.failed)
+                        throw hip_failure("an insert found no room in the overflow table");
+                    inserted += c.inserted, size += c.inserted, ov_count += c.ov_inserted;
+                    grow(sched_add(s, c.inserted));
+                }
+                if(!staged)
+                    more = stage_next_chunk();
+            };
+            if(seqs)
+                host_pipeline(seqs, lens, n, work);
+            else
+                text_pipeline(host_text, text_bytes, work);
         }
         catch(...)
         {
@@ -843,6 +978,12 @@ struct kache_table
 
     uint64_t update_text(uint32_t op, const uint8_t* text, uint64_t text_len, uint64_t x) const
     {
+        launch_update_op(op, text, text_len, x);
+        return read_counters().present;
+    }
+
+    void launch_update_op(uint32_t op, const uint8_t* text, uint64_t text_len, uint64_t x) const
+    {
         timed("update", [&]{
             switch(op)
             {
@@ -855,18 +996,23 @@ struct kache_table
             default:        launch_update<OP_ADD_WRAP>(text, text_len, 0, text_len, x); break;
             }
         });
-        return read_counters().present;
     }
 
-    uint64_t update_host(const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x)
+    uint64_t update_host(const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x,
+                         const char* host_text = nullptr, uint64_t text_bytes = 0)
     {
         std::lock_guard<std::mutex> lock(mu);
-        uint64_t present = 0;
-        SeqCursor cur;
-        Stage st;
-        while(stage_next(seqs, lens, n, cur, st))
-            present += update_text(op, upload(st), st.text_len, x);
-        return present;
+        // Counts add up in the device counters; one read at the end.
+        const auto work = [&](const uint8_t* text, uint64_t text_len, const auto& stage_next_chunk, bool& more)
+        {
+            launch_update_op(op, text, text_len, x);
+            more = stage_next_chunk();
+        };
+        if(seqs)
+            host_pipeline(seqs, lens, n, work);
+        else
+            text_pipeline(host_text, text_bytes, work);
+        return read_counters().present;
     }
 
     // --- Queries ---
@@ -1130,6 +1276,33 @@ struct kache_table
         return r;
     }
 
+    // Histogram of the values in [1, max] (counter/Kmer_Set_Counter.cpp:97-120):
+    // out[c] = keys with value c, out[0] = 0.
+    void histogram(uint64_t max, uint64_t* out) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        DevBuf d;
+        d.reserve((max + 1) * 8);
+        HIP(hipMemsetAsync(d.p, 0, (max + 1) * 8, stream));
+        const uint64_t blocks = (cap * B + ov_slots + TILE - 1) / TILE;
+        const uint64_t lds = (max + 1) * 4;
+        timed("histogram", [&]{
+            if(lds <= 48 * 1024)
+            {
+                int per_cu = 0;
+                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_histogram<true>, TILE, lds) != hipSuccess || per_cu < 1)
+                    per_cu = 1;
+                const uint64_t g = std::min<uint64_t>(blocks, uint64_t(per_cu) * device_cus());
+                hipLaunchKernelGGL(k_histogram<true>, dim3(std::max<uint64_t>(g, 1)), dim3(TILE), lds, stream, view(), max, d.as<uint64_t>());
+            }
+            else
+                hipLaunchKernelGGL(k_histogram<false>, dim3(grid_for(k_histogram<false>, blocks)), dim3(TILE), 0, stream, view(), max, d.as<uint64_t>());
+        });
+        HIP(hipMemcpyAsync(out, d.p, (max + 1) * 8, hipMemcpyDeviceToHost, stream));
+        sync();
+        out[0] = 0;
+    }
+
     // capi/Kmer_Table.hpp:270-279 with the placement of
     // include/Streaming_Kmer_Hash_Table.hpp:2728-2752.
     void stats(kache_stats& s) const
@@ -1348,6 +1521,40 @@ int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const
         t->lookup_groups(seqs, lens, n, d.data(), n_groups);
         return static_cast<int>(KACHE_OK);
     });
+}
+
+int kache_amd_insert_text(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) noexcept
+{
+    if(t == nullptr || !registered(t, user) || (text == nullptr && text_bytes > 0) ||
+       (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = text_bytes ? t->insert_host(user, nullptr, nullptr, 0, t->map ? op : 0, false, t->map ? x : 0, 0, text, text_bytes) : 0;
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_update_text(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) noexcept
+{
+    if(t == nullptr || (text == nullptr && text_bytes > 0) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = text_bytes ? t->update_host(nullptr, nullptr, 0, op, x, text, text_bytes) : 0;
+        if(present != nullptr)
+            *present = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_histogram(const kache_table* t, uint64_t max, uint64_t* out) noexcept
+{
+    if(t == nullptr || out == nullptr || t->value_bytes == 0 || max > (uint64_t(1) << 30))
+        return KACHE_EINVAL;
+    return guard([&](){ t->histogram(max, out); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_get(const kache_table* t, const char* kmer, uint8_t* found, uint64_t* value) noexcept
