@@ -47,7 +47,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--seed", type=int, default=None)
-    p.add_argument("--workload", choices=["count-map", "serve"], default="count-map")
+    p.add_argument("--workload", choices=["count-map", "serve", "counter"], default="count-map")
+    p.add_argument("--counter-genome", type=int, default=950_000_000)
+    p.add_argument("--counter-read-gbp", type=float, default=30.0)
     p.add_argument("--requests", type=int, default=2000, help="serve: requests per run")
     p.add_argument("--serve-genome", type=int, default=2_000_000_000)
     p.add_argument("--serve-expected", type=int, default=2_000_000_000)
@@ -132,6 +134,8 @@ def main():
     api = KA.api()
     if a.workload == "serve":
         return serve_main(a, KA, capi, api)
+    if a.workload == "counter":
+        return counter_main(a, KA, capi, api)
     if a.seed is None:
         a.seed = 0xC2
     nb = (a.reads + a.batch - 1) // a.batch
@@ -465,6 +469,173 @@ def serve_main(a, KA, capi, api):
         "capacity": {"served": cap_served, "plain_per_request": cap_plain},
         "sweep": sweep,
         "table": {"keys": t.size(), "buckets": info["buckets"], "resizes": info["resizes"], "device_bytes": info["device_bytes"]},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+# --- Counter workload -------------------------------------------------------
+
+def counter_inputs(capi, api, genome_len, read_gbp, seed, donor_frac=0.6):
+    """The counter's inputs, in host memory: segments covering a seeded
+    genome (lengths uniform in [1, 50,000]) plus 5% more bases of segment
+    variants (copies of random places with one substitution per 500 bases),
+    newline-separated; and reads of 150 bp (0.5% substitutions, 0.05% N)
+    from the first `donor_frac` of the genome, newline-separated."""
+    rng = np.random.default_rng(seed)
+    g = capi.DeviceBuffer(api, genome_len)
+    api.check(api.lib.kache_amd_synth_genome(g.ptr, genome_len, seed), "synth_genome")
+    genome = g.download(genome_len)
+    g.free()
+    cuts = np.cumsum(rng.integers(1, 50_001, genome_len // 25_000 + 10))
+    cuts = cuts[cuts < genome_len]
+    parts = np.split(genome, cuts)
+    extra, want = [], genome_len // 20
+    while want > 0:
+        n = int(min(want, rng.integers(1, 50_001)))
+        a0 = int(rng.integers(0, genome_len - n))
+        piece = genome[a0:a0 + n].copy()
+        sub = np.arange(int(rng.integers(0, 500)), n, 500)
+        piece[sub] = _ACGT[(_CODE[piece[sub]] + 1) % 4]
+        extra.append(piece)
+        want -= n
+    nl = np.frombuffer(b"\n", np.uint8)
+    segs = parts + extra
+    prime = np.concatenate([x for p in segs for x in (p, nl)])
+    L1 = READ_LEN + 1
+    n_reads = int(read_gbp * 1e9) // READ_LEN
+    reads = np.empty(n_reads * L1, np.uint8)
+    chunk = 10_000_000
+    buf = capi.DeviceBuffer(api, chunk * L1)
+    donor = int(genome_len * donor_frac)
+    for r0 in range(0, n_reads, chunk):
+        c = min(chunk, n_reads - r0)
+        # Each chunk is its own seeded read set from the donor part of the genome.
+        api.check(api.lib.kache_amd_synth_reads(buf.ptr, c, READ_LEN, seed, donor, seed + 1 + r0 // chunk, 0.005, 0.0005, 0.0), "synth_reads")
+        reads[r0 * L1:(r0 + c) * L1] = buf.download(c * L1)
+    buf.free()
+    return prime, reads, len(segs)
+
+
+def counter_cpu(capi, prime, reads, n_prime_bytes, n_read_bytes, threads):
+    """The reference C ABI (oracle/_ref) as the reference's counter drives its
+    table (counter/Kmer_Set_Counter.cpp:64-94): a pow2 u32 table at load
+    0.9; prime = upsert KEEP 0 of the segments, count = update +1 of the
+    first reads, from `threads` threads in batches of 1,024 sequences.
+    Timed: the count.  Returns k-mer windows/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    path = O.ref_library()
+    if path is None:
+        return None
+    api = capi.Api(path)
+    p = prime[:n_prime_bytes]
+    p = p[:int(np.flatnonzero(p == 10)[-1]) + 1]
+    t = capi.Table(api, 4, len(p), 0.9, True, threads)
+    users = [t.register() for _ in range(threads)]
+
+    def batches(text, per=1024):
+        ends = np.flatnonzero(text == 10)
+        starts = np.concatenate([[0], ends[:-1] + 1])
+        out = []
+        for i in range(0, len(ends), per):
+            s, e = starts[i:i + per], ends[i:i + per]
+            out.append(((s + np.uint64(text.ctypes.data)).astype(np.uint64), (e - s).astype(np.uint64)))
+        return out
+
+    def run(bs, fn):
+        def work(w):
+            n = C.c_uint64()
+            for j in range(w, len(bs), threads):
+                ptrs, lens = bs[j]
+                assert fn(w, ptrs, lens, n) == 0
+        th = [threading.Thread(target=work, args=(w,)) for w in range(threads)]
+        t0 = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        return time.perf_counter() - t0
+
+    run(batches(p), lambda w, ptrs, lens, n: api.lib.kache_insert(t.h, users[w], ptrs.ctypes.data, lens.ctypes.data, len(lens), capi.KEEP, 0, C.byref(n)))
+    r = reads[:n_read_bytes]
+    r = r[:int(np.flatnonzero(r == 10)[-1]) + 1]
+    rb = batches(r)
+    dt = run(rb, lambda w, ptrs, lens, n: api.lib.kache_update(t.h, ptrs.ctypes.data, lens.ctypes.data, len(lens), capi.ADD, 1, C.byref(n)))
+    pos = sum(int(np.maximum(l.astype(np.int64) - 30, 0).sum()) for _, l in rb)
+    size = t.size()
+    t.close()
+    return {"value": pos / dt, "unit": "k-mer count updates/s", "cores": threads, "kind": "reference",
+            "sample": f"prime {len(p) >> 20} MiB of the segments ({size} k-mers), then count {len(r) >> 20} MiB of the reads "
+                      f"({pos} positions) by {threads} threads in batches of 1,024 sequences; {os.path.basename(path)}",
+            "seconds": dt}
+
+
+def counter_main(a, KA, capi, api):
+    """Set counting as the reference's Kmer_Set_Counter does it, on one GPU:
+    prime a u32 set (pow2 table, load 0.9) with 1 Gbp of segments, count
+    30 Gbp of reads streamed from host memory, histogram(10000)."""
+    seed = 0xC4 if a.seed is None else a.seed
+    t0 = time.perf_counter()
+    prime, reads, n_segs = counter_inputs(capi, api, a.counter_genome, a.counter_read_gbp, seed)
+    print(f"counter: {len(prime) >> 20} MiB of {n_segs} segments, {len(reads) >> 20} MiB of reads in host memory "
+          f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    t = KA.table(width=4, expected_keys=a.counter_genome, load_factor=0.9, pow2=True)
+    u = t.register()
+    t.profile(True)
+    t1 = time.perf_counter()
+    t.insert_text(prime, KA.KEEP, 0, u)
+    dt_prime = time.perf_counter() - t1
+    size = t.size()
+    # Counting: one call per ~4 GiB of whole reads, as a reader hands them over.
+    step = (4 << 30) // (READ_LEN + 1) * (READ_LEN + 1)
+    found = 0
+    t2 = time.perf_counter()
+    for b0 in range(0, len(reads), step):
+        b1 = min(len(reads), b0 + step)
+        found += t.update_text(reads[b0:b1], KA.ADD_WRAP, 1)
+    dt_count = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    hist = t.histogram(10_000)
+    dt_hist = time.perf_counter() - t3
+    upd_ms, upd_launches = t.kernel_time("update")
+    t.profile(False)
+    # k-mer windows of the reads (those with an N, about 1.5%, are skipped by the update)
+    positions = (len(reads) // (READ_LEN + 1)) * (READ_LEN - 30)
+    h = hist.astype(np.float64)
+    zero = size - int(hist.sum())
+    checks = {"sum_c_hist_equals_found": int((np.arange(len(hist), dtype=np.uint64) * hist).sum()) == found,
+              "keys_at_zero_pct": 100.0 * zero / max(size, 1)}
+    stats = t.stats()
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"counter: reference CPU path, {threads} threads", file=sys.stderr, flush=True)
+        cpu = counter_cpu(capi, prime, reads, len(prime), 1 << 30, threads)
+    # The update kernel's roofline: 1.25 B of input + the 64-byte metadata line per
+    # 5.29 k-mers + the 12-byte u32 slot, per k-mer position.
+    alg = 150 / 120 + 64 / 5.29 + 12
+    achieved = (positions / max(upd_launches, 1)) * alg / (upd_ms / max(upd_launches, 1) * 1e-3) / 1e9 if upd_ms > 0 else 0.0
+    out = {
+        "metric": "k-mer upserts/s + lookups/s (k=31 count map); p50/p99 served-batch latency",
+        "value": positions / dt_count, "unit": "k-mer count updates/s", "n_gpus": 1,
+        "steps": (len(reads) + step - 1) // step, "warmup": 0, "ms_per_step": dt_count * 1e3 / ((len(reads) + step - 1) // step),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded genome and reads made on device, held in host memory)",
+        "config": {"workload": "set counter <31,u32,22> (pow2, load 0.9): prime with segments of a 950 Mbp genome (lengths "
+                               "U[1, 50k]) plus 5% variant segments; count 30 Gbp of 150-bp reads (0.5% subst, 0.05% N) "
+                               "from the first 60% of the genome, streamed from host memory; histogram(10000)",
+                   "genome_bp": a.counter_genome, "read_bytes": len(reads), "prime_bytes": len(prime)},
+        "prime_kmers_per_s": (len(prime) - n_segs * 31) / dt_prime, "prime_seconds": dt_prime,
+        "count_seconds": dt_count, "histogram_seconds": dt_hist, "found": found, "set_size": size,
+        "capacity": stats["capacity"], "checks": checks,
+        "host_to_device_GBps": len(reads) / dt_count / 1e9,
+        "kernel_ms": {"update": upd_ms, "update_launches": upd_launches},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "k_update<ADD_WRAP>",
+                     "alg_bytes_per_kmer": alg},
         "cpu_baseline": cpu,
     }
     if cpu:

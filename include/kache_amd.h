@@ -69,6 +69,28 @@ int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text
 int kache_amd_insert_text(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
 int kache_amd_update_text(kache_table* t, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
 
+/* kache_insert / kache_update that also take KACHE_AMD_ADD_WRAP (the C++
+ * API's wrapping v + x); otherwise the same calls. */
+int kache_amd_insert_seqs(kache_table* t, uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_update_seqs(kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
+
+/* kache_amd_insert_text / kache_amd_update_text over the sequences of a FASTA
+ * or FASTQ file, read as the reference's counter reads its inputs
+ * (counter/Fastx_Reader.hpp:17-22): the first byte sets the format, FASTA
+ * lines join up to the next header, FASTQ records have four lines with a
+ * quality line as long as the sequence, lines end with "\n" or "\r\n".
+ * A file that cannot be opened, is gzip-compressed, or is malformed gives
+ * EINVAL (kache_amd_last_error says why; records before a malformed one may
+ * have been applied).  Blocks of about 64 MiB of sequence each make one
+ * text call. */
+typedef int (*kache_amd_text_fn)(void* ctx, const char* text, uint64_t bytes);
+/* The reader alone (host only): calls fn(ctx, text, bytes) on each block of
+ * the file's sequences, each followed by '\n'; a nonzero return stops it
+ * (EINVAL). */
+int kache_amd_read_fastx(const char* path, kache_amd_text_fn fn, void* ctx) KACHE_NOEXCEPT;
+int kache_amd_insert_fastx(kache_table* t, uint32_t user, const char* path, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
+int kache_amd_update_fastx(kache_table* t, const char* path, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
+
 /* Histogram of a map's values, as the reference's counter computes it
  * (counter/Kmer_Set_Counter.cpp:97-120): out[c] = the number of keys with
  * value c for c in [1, max], out[0] = 0; `out` holds max + 1 entries.  One
@@ -206,6 +228,10 @@ uint64_t kache_amd_host_minimizer_hash(uint64_t key) KACHE_NOEXCEPT;
 uint8_t kache_amd_host_checksum(uint64_t key) KACHE_NOEXCEPT;
 uint64_t kache_amd_host_xxh3(uint64_t x, uint64_t seed) KACHE_NOEXCEPT;
 uint64_t kache_amd_host_cap_for(uint64_t n, double lf, uint32_t pow2) KACHE_NOEXCEPT;
+/* Whether a table for n keys at load factor lf (pow2: bucket count rounded up
+ * to a power of 2) is within the most buckets supported
+ * (include/Streaming_Kmer_Hash_Table.hpp:799-804). */
+int kache_amd_host_fits(uint64_t n, double lf, uint32_t pow2) KACHE_NOEXCEPT;
 /* Describes a host text the way one device tile does (packing, windows and
  * window minima), position by position; same outputs as the device call. */
 int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) KACHE_NOEXCEPT;

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkache_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kache_table.hip", "kache_synth.hip", "kache_serve.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kache_table.hip", "kache_synth.hip", "kache_serve.cpp", "kache_fastx.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("kache_device.hpp", "kache_kernels.hpp")] + [
     os.path.join(ROOT, "include", f) for f in ("kache_capi.h", "kache_amd.h")
 ]
@@ -37,6 +37,8 @@ def command(out=LIB, extra=()):
 
 
 ROOF = os.path.join(HERE, "memory_roof")
+CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "cpp_api_check.cpp")
+CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "cpp_api_check")
 
 
 def build(verbose=True):
@@ -54,6 +56,13 @@ def build(verbose=True):
         print(" ".join(roof), flush=True)
     subprocess.run(roof, check=True, cwd=ROOT)
     os.replace(ROOF + ".tmp", ROOF)
+    # The C++ API's check program (host code over the library).
+    cpp = ["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_CHECK_SRC,
+           "-L", HERE, "-lkache_amd", "-Wl,-rpath,$ORIGIN/../../kache_hash_amd", "-o", CPP_CHECK + ".tmp"]
+    if verbose:
+        print(" ".join(cpp), flush=True)
+    subprocess.run(cpp, check=True, cwd=ROOT)
+    os.replace(CPP_CHECK + ".tmp", CPP_CHECK)
     return LIB
 
 

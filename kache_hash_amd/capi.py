@@ -57,6 +57,7 @@ class ServerStats(C.Structure):
 
 
 KEYS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_size_t)
+TEXT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_uint64)
 
 _P = C.c_void_p
 _SEQS = C.c_void_p   # const char* const*
@@ -100,6 +101,11 @@ AMD_SYMBOLS = {
     "kache_amd_insert_text": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_update_text": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_histogram": (C.c_int, [_P, C.c_uint64, _P]),
+    "kache_amd_insert_seqs": (C.c_int, [_P, C.c_uint32, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_update_seqs": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_read_fastx": (C.c_int, [C.c_char_p, TEXT_FN, _P]),
+    "kache_amd_insert_fastx": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kache_amd_update_fastx": (C.c_int, [_P, C.c_char_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_synth_genome": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
     "kache_amd_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "kache_amd_host_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
@@ -121,6 +127,7 @@ AMD_SYMBOLS = {
     "kache_amd_host_checksum": (C.c_uint8, [C.c_uint64]),
     "kache_amd_host_xxh3": (C.c_uint64, [C.c_uint64, C.c_uint64]),
     "kache_amd_host_cap_for": (C.c_uint64, [C.c_uint64, C.c_double, C.c_uint32]),
+    "kache_amd_host_fits": (C.c_int, [C.c_uint64, C.c_double, C.c_uint32]),
     "kache_amd_host_describe": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P]),
 }
 
@@ -377,6 +384,16 @@ class Table:
         self.api.check(self.api.lib.kache_amd_update_text(self.h, buf.ctypes.data if buf.size else None, buf.size, op, x, C.byref(n)), "kache_amd_update_text")
         return n.value
 
+    def insert_fastx(self, path, op=KEEP, x=0, user=0):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_insert_fastx(self.h, user, os.fsencode(path), op, x, C.byref(n)), "kache_amd_insert_fastx")
+        return n.value
+
+    def update_fastx(self, path, op, x):
+        n = C.c_uint64()
+        self.api.check(self.api.lib.kache_amd_update_fastx(self.h, os.fsencode(path), op, x, C.byref(n)), "kache_amd_update_fastx")
+        return n.value
+
     def histogram(self, max_count):
         """out[c] = keys with value c, c in [1, max_count]; out[0] = 0."""
         out = np.zeros(max_count + 1, np.uint64)
@@ -511,6 +528,21 @@ class Server:
         s = ServerStats()
         self.api.check(self.api.lib.kache_amd_server_get_stats(self.h, C.byref(s)), "kache_amd_server_get_stats")
         return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+
+def read_fastx(api, path):
+    """The sequences of a FASTA/FASTQ file as the library reads them: a list
+    of bytes.  Raises KacheError on malformed input."""
+    blocks = []
+
+    def cb(ctx, p, n):
+        blocks.append(C.string_at(p, n))
+        return 0
+
+    fn = TEXT_FN(cb)
+    api.check(api.lib.kache_amd_read_fastx(os.fsencode(path), fn, None), "kache_amd_read_fastx")
+    text = b"".join(blocks)
+    return text.split(b"\n")[:-1] if text else []
 
 
 def device_text(api, seqs):

@@ -1523,6 +1523,33 @@ int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const
     });
 }
 
+int kache_amd_insert_seqs(kache_table* t, uint32_t user, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* inserted) noexcept
+{
+    if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) ||
+       (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = t->insert_host(user, seqs, lens, n, t->map ? op : 0, false, t->map ? x : 0, 0);
+        if(inserted != nullptr)
+            *inserted = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_update_seqs(kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint32_t op, uint64_t x, uint64_t* present) noexcept
+{
+    if(t == nullptr || !valid(seqs, lens, n) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        const uint64_t c = t->update_host(seqs, lens, n, op, x);
+        if(present != nullptr)
+            *present = c;
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
 int kache_amd_insert_text(kache_table* t, uint32_t user, const char* text, uint64_t text_bytes, uint32_t op, uint64_t x, uint64_t* inserted) noexcept
 {
     if(t == nullptr || !registered(t, user) || (text == nullptr && text_bytes > 0) ||
@@ -1788,6 +1815,11 @@ uint8_t kache_amd_host_checksum(uint64_t key) noexcept { return checksum(key); }
 uint64_t kache_amd_host_xxh3(uint64_t x, uint64_t seed) noexcept { return xxh3_8(x, seed); }
 uint64_t kache_amd_host_cap_for(uint64_t n, double lf, uint32_t pow2) noexcept { return cap_for(n, lf, pow2 != 0); }
 
+int kache_amd_host_fits(uint64_t n, double lf, uint32_t pow2) noexcept
+{
+    return lf > 0 && lf < 1 && size_fits(n, lf, pow2 != 0);
+}
+
 int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) noexcept
 {
     if(text_bytes > 0 && (!text || !key || !h || !c || !fwd || !valid))
@@ -1844,4 +1876,10 @@ int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key
     return KACHE_OK;
 }
 
+}
+
+// Sets the calling thread's last error (other sources of the library).
+__attribute__((visibility("hidden"))) void kamd_set_last_error(const std::string& what)
+{
+    last_error = what;
 }

@@ -149,3 +149,26 @@ def test_server_rejects_invalid_arguments(api):
     assert L.kache_amd_server_get_stats(None, None) == E
     assert L.kache_amd_server_replay(None, None, None, 0, None) == E
     assert L.kache_amd_clock() > 0
+
+
+def test_cpp_headers_compile(tmp_path):
+    """The C++ API headers compile on their own (C++20, -Wall -Werror) and the
+    C headers as C."""
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "kache_amd.hpp"\n#include "kache_amd_counter.hpp"\n'
+                   'int main(){ using M = kache_amd::Streaming_Kmer_Hash_Table<31, true, uint16_t, 22>;'
+                   ' using S = kache_amd::Streaming_Kmer_Hash_Table<31, true, void, 22>;'
+                   ' return M::fits(1000, 0.8, false) && S::fits(1000, 0.8, true) ? 0 : 1; }\n')
+    import subprocess
+    subprocess.run(["g++", "-std=c++20", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(src)], check=True)
+    c = tmp_path / "t.c"
+    c.write_text('#include "kache_amd.h"\nint main(void){ return kache_supports(31, 4) ? 0 : 1; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(c)], check=True)
+
+
+def test_host_fits_matches_oracle(api):
+    for n in (0, 1000, 10**9, 137 * 10**9, 2**40):
+        for lf in (0.5, 0.8, 0.9):
+            for pow2 in (0, 1):
+                assert bool(api.lib.kache_amd_host_fits(n, lf, pow2)) == O.fits(n, lf, bool(pow2)), (n, lf, pow2)
