@@ -47,7 +47,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--seed", type=int, default=None)
-    p.add_argument("--workload", choices=["count-map", "serve", "counter"], default="count-map")
+    p.add_argument("--workload", choices=["count-map", "serve", "counter", "batched"], default="count-map")
+    p.add_argument("--batched-gbp", type=float, default=8.0)
+    p.add_argument("--batched-genome", type=int, default=500_000_000)
+    p.add_argument("--batched-expected", type=int, default=187_500_000)
+    p.add_argument("--cpu-sample-bytes", type=int, default=1 << 30)
     p.add_argument("--counter-genome", type=int, default=950_000_000)
     p.add_argument("--counter-read-gbp", type=float, default=30.0)
     p.add_argument("--requests", type=int, default=2000, help="serve: requests per run")
@@ -136,6 +140,8 @@ def main():
         return serve_main(a, KA, capi, api)
     if a.workload == "counter":
         return counter_main(a, KA, capi, api)
+    if a.workload == "batched":
+        return batched_main(a, KA, capi, api)
     if a.seed is None:
         a.seed = 0xC2
     nb = (a.reads + a.batch - 1) // a.batch
@@ -636,6 +642,150 @@ def counter_main(a, KA, capi, api):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "k_update<ADD_WRAP>",
                      "alg_bytes_per_kmer": alg},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+# --- Batched uneven sequences -------------------------------------------------
+
+def batched_text(capi, api, gbp, genome_len, seed):
+    """Sequences of lognormal lengths (median 2 kbp, sigma 1.3, clamped to
+    [50, 100,000]) totalling `gbp` Gbp, from a seeded genome with 1%
+    substitutions, 0.05% N and lowercase runs in 20% of them, as one device
+    text.  Returns (buffer, text bytes, number of sequences)."""
+    rng = np.random.default_rng(seed)
+    target = int(gbp * 1e9)
+    lens = []
+    total = 0
+    while total < target:
+        l = np.clip(np.rint(rng.lognormal(np.log(2000), 1.3, 1 << 20)), 50, 100_000).astype(np.uint64)
+        c = np.cumsum(l)
+        k = int(np.searchsorted(c, target - total)) + 1
+        lens.append(l[:k])
+        total += int(c[min(k, len(c)) - 1])
+    lens = np.concatenate(lens)
+    n = len(lens)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens + 1, out=off[1:])
+    text_bytes = int(off[-1])
+    buf = capi.DeviceBuffer(api, text_bytes + capi.TEXT_PAD)
+    d_off = capi.DeviceBuffer(api, 8 * (n + 1))
+    d_off.upload(off.view(np.uint8))
+    api.check(api.lib.kache_amd_synth_seqs(buf.ptr, d_off.ptr, n, int(lens.max()), seed, genome_len, seed + 1, 0.01, 0.0005, 0.2), "synth_seqs")
+    api.check(api.lib.kache_amd_device_fill(buf.ptr + text_bytes, ord("\n"), capi.TEXT_PAD), "fill")
+    d_off.free()
+    return buf, text_bytes, n
+
+
+def batched_cpu(capi, host, expected, threads, oriented):
+    """The reference C ABI (oracle/_ref) inserting the same sequences (a
+    prefix of the text) from `threads` threads, each its own user, in calls
+    of about 1 Mbp (as the reference's reader batches them); the table
+    starts at `expected` keys and doubles as it fills."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    path = O.ref_library()
+    if path is None:
+        return None
+    api = capi.Api(path)
+    t = capi.Table(api, 1 if oriented else 4, expected, 0.8, False, threads)
+    users = [t.register() for _ in range(threads)]
+    ends = np.flatnonzero(host == 10)
+    starts = np.concatenate([[0], ends[:-1] + 1])
+    calls, a = [], 0
+    csum = np.cumsum(ends - starts)
+    while a < len(ends):
+        b = int(np.searchsorted(csum, (csum[a - 1] if a else 0) + (1 << 20))) + 1
+        b = min(max(b, a + 1), len(ends))
+        calls.append(((starts[a:b] + np.uint64(host.ctypes.data)).astype(np.uint64), (ends[a:b] - starts[a:b]).astype(np.uint64)))
+        a = b
+
+    def work(w):
+        n = C.c_uint64()
+        for j in range(w, len(calls), threads):
+            ptrs, lens = calls[j]
+            if oriented:
+                st = api.lib.kache_insert_oriented(t.h, users[w], ptrs.ctypes.data, lens.ctypes.data, len(lens), capi.OR, 1, 2, C.byref(n))
+            else:
+                st = api.lib.kache_insert(t.h, users[w], ptrs.ctypes.data, lens.ctypes.data, len(lens), capi.ADD, 1, C.byref(n))
+            assert st == 0
+
+    th = [threading.Thread(target=work, args=(w,)) for w in range(threads)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.perf_counter() - t0
+    pos = sum(int(np.maximum(l.astype(np.int64) - 30, 0).sum()) for _, l in calls)
+    stats = t.stats()
+    t.close()
+    return {"value": pos / dt, "unit": "k-mer upserts/s", "cores": threads, "kind": "reference",
+            "sample": f"the first {len(host) >> 20} MiB of the text ({pos} k-mer windows) inserted by {threads} threads in calls "
+                      f"of ~1 Mbp into a {'u8 oriented OR' if oriented else 'u32 +1'} map starting at {expected} keys "
+                      f"(capacity after: {stats['capacity']}); {os.path.basename(path)}",
+            "seconds": dt}
+
+
+def batched_main(a, KA, capi, api):
+    """Uneven sequences in persistent launches with growth: a u32 count map
+    (+1) and a u8 oriented map (OR 1 forward / 2 reverse), each from 1/16 of
+    the final size, over 8 Gbp of lognormal-length sequences; the launch
+    shape is chosen online by the autotuner."""
+    seed = 0xC3 if a.seed is None else a.seed
+    t0 = time.perf_counter()
+    text, nbytes, nseq = batched_text(capi, api, a.batched_gbp, a.batched_genome, seed)
+    print(f"batched: {nseq} sequences, {nbytes} bytes on the device ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    results = {}
+    for name, width, oriented in (("u32_count", 4, False), ("u8_oriented", 1, True)):
+        t = KA.table(width=width, expected_keys=a.batched_expected)
+        u = t.register()
+        t.autotune(2)
+        t.profile(True)
+        t1 = time.perf_counter()
+        if oriented:
+            t.insert_oriented_device(text.ptr, nbytes, KA.OR, 1, 2, u)
+        else:
+            t.insert_device(text.ptr, nbytes, KA.ADD, 1, u)
+        dt = time.perf_counter() - t1
+        ins_ms, ins_n = t.kernel_time("insert")
+        split_ms, _ = t.kernel_time("split")
+        re_ms, _ = t.kernel_time("reinsert")
+        t.profile(False)
+        _, kmers = t.count_device(text.ptr, nbytes, positions=True)
+        info, stats = t.info(), t.stats()
+        results[name] = {"kmers_per_s": kmers / dt, "seconds": dt, "kmers": kmers, "keys": stats["keys"],
+                         "capacity": stats["capacity"], "resizes": info["resizes"], "device_bytes": info["device_bytes"],
+                         "kernel_ms": {"insert": ins_ms, "insert_launches": ins_n, "split": split_ms, "reinsert": re_ms},
+                         "tuning": t.tuning(), "trials": t.tuning_trials()}
+        print(f"batched {name}: {kmers / dt / 1e9:.2f} G k-mers/s, {info['resizes']} doublings, tuned {t.tuning()}", file=sys.stderr, flush=True)
+        t.close()
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        host = text.download(min(nbytes, a.cpu_sample_bytes))
+        host = host[:int(np.flatnonzero(host == 10)[-1]) + 1]
+        print(f"batched: reference CPU path on {len(host) >> 20} MiB, {threads} threads", file=sys.stderr, flush=True)
+        cpu = batched_cpu(capi, host, max(len(host) // 16, 1 << 20), threads, False)
+    u32 = results["u32_count"]
+    alg = 1.0 + 64 / 5.49 + 12      # long sequences: input, the metadata line per minimizer run, the 12-byte u32 slot
+    ms, n = u32["kernel_ms"]["insert"], u32["kernel_ms"]["insert_launches"]
+    achieved = (u32["kmers"] / max(n, 1)) * alg / (ms / max(n, 1) * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {
+        "metric": "k-mer upserts/s + lookups/s (k=31 count map); p50/p99 served-batch latency",
+        "value": u32["kmers_per_s"], "unit": "k-mer upserts/s (u32 count map, doublings included)", "n_gpus": 1,
+        "steps": n, "warmup": 0, "ms_per_step": u32["seconds"] * 1e3 / max(n, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded on device)",
+        "config": {"workload": "batched uneven sizes with growth: 8 Gbp of lognormal-length sequences (median 2 kbp, "
+                               "sigma 1.3, 50 bp-100 kbp) from a 500 Mbp genome, 1% subst; <31,u32,22> count map and "
+                               "<31,u8,22> oriented OR map, each from 1/16 of the final size; autotuned launch shape",
+                   "sequences": nseq, "text_bytes": nbytes, "expected_keys": a.batched_expected},
+        "tables": results,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "k_insert<map, ADD> (u32)", "alg_bytes_per_kmer": alg},
         "cpu_baseline": cpu,
     }
     if cpu:

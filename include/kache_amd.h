@@ -112,6 +112,37 @@ int kache_amd_kernel_time(const kache_table* t, const char* name, double* ms, ui
 /* Blocks per compute unit of the persistent launches (0: as many as fit). */
 int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
 
+/* Launch shape of the streaming inserts of kache_amd_insert_device and
+ * kache_amd_insert_oriented_device: blocks per compute unit (0: as many as
+ * fit), positions per launch (2^launch_log2, 16..30), and whether a launch
+ * and its counter read-back replay a captured hipGraph (graph = 1) instead
+ * of being enqueued one by one.  The shape changes speed only, never
+ * results. */
+typedef struct kache_amd_tuning
+{
+    uint32_t blocks_per_cu;
+    uint32_t launch_log2;
+    uint32_t graph;
+    uint32_t tuning;            /* get: rounds left to the tuner, 0 when done */
+} kache_amd_tuning;
+int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) KACHE_NOEXCEPT;
+int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) KACHE_NOEXCEPT;
+
+/* Online autotuner: the next insert launches of those calls try each
+ * candidate shape (blocks per CU: all, 3/4, 1/2 of what fits; positions per
+ * launch 2^23, 2^25; graph replay off, on) in turn, `rounds` launches each,
+ * timed on the host; then the shape with the least median time per position
+ * stays.  The launches are real work: nothing is repeated.  rounds = 0
+ * stops it; at most 16. */
+int kache_amd_autotune(kache_table* t, uint32_t rounds) KACHE_NOEXCEPT;
+typedef struct kache_amd_tuning_trial
+{
+    kache_amd_tuning cfg;
+    uint32_t launches;
+    double median_ns_per_position;
+} kache_amd_tuning_trial;
+int kache_amd_tuning_trials(const kache_table* t, kache_amd_tuning_trial* out, size_t cap, size_t* n) KACHE_NOEXCEPT;
+
 /* Table geometry and history. */
 typedef struct kache_amd_info
 {
@@ -145,6 +176,13 @@ int kache_amd_synth_reads(char* out, uint64_t n_reads, uint64_t read_len, uint64
                           uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
 int kache_amd_host_synth_reads(char* out, uint64_t first_read, uint64_t n_reads, uint64_t read_len, uint64_t genome_seed,
                                uint64_t genome_len, uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
+/* Sequences of uneven lengths as a device text: sequence i fills
+ * [offsets[i], offsets[i + 1] - 1) and '\n' ends it; `offsets` (n + 1
+ * entries, device memory) must give lengths in [1, max_len], max_len <=
+ * genome_len.  Each is a window of the seeded genome at a uniform start,
+ * with the read model above. */
+int kache_amd_synth_seqs(char* out, const uint64_t* offsets, uint64_t n, uint64_t max_len, uint64_t genome_seed, uint64_t genome_len,
+                         uint64_t seed, double sub, double n_rate, double lower) KACHE_NOEXCEPT;
 const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
 
 /* Serving loop over a resident table.  Requests are submitted from any

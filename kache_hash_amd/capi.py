@@ -41,6 +41,14 @@ class Info(C.Structure):
 REQ_COUNT, REQ_LOOKUP, REQ_UPDATE, REQ_INSERT = range(4)   # KACHE_AMD_REQ_*
 
 
+class Tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("launch_log2", C.c_uint32), ("graph", C.c_uint32), ("tuning", C.c_uint32)]
+
+
+class TuningTrial(C.Structure):
+    _fields_ = [("cfg", Tuning), ("launches", C.c_uint32), ("median_ns_per_position", C.c_double)]
+
+
 class Request(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("op", C.c_uint32), ("x", C.c_uint64), ("seqs", C.c_void_p), ("lens", C.c_void_p),
                 ("n", C.c_size_t), ("counts", C.c_void_p), ("found", C.c_void_p), ("values", C.c_void_p), ("fwd", C.c_void_p)]
@@ -110,6 +118,11 @@ AMD_SYMBOLS = {
     "kache_amd_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "kache_amd_host_synth_reads": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "kache_amd_synth_error": (C.c_char_p, []),
+    "kache_amd_synth_seqs": (C.c_int, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double]),
+    "kache_amd_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "kache_amd_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "kache_amd_autotune": (C.c_int, [_P, C.c_uint32]),
+    "kache_amd_tuning_trials": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "kache_amd_describe_device": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P]),
     "kache_amd_profile": (C.c_int, [_P, C.c_int]),
     "kache_amd_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -408,6 +421,26 @@ class Table:
         n = C.c_uint64()
         self.api.check(self.api.lib.kache_amd_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)), "kache_amd_kernel_time")
         return ms.value, n.value
+
+    def set_tuning(self, blocks_per_cu=0, launch_log2=24, graph=False):
+        cfg = Tuning(blocks_per_cu, launch_log2, int(graph), 0)
+        self.api.check(self.api.lib.kache_amd_set_tuning(self.h, C.byref(cfg)), "kache_amd_set_tuning")
+
+    def tuning(self):
+        cfg = Tuning()
+        self.api.check(self.api.lib.kache_amd_get_tuning(self.h, C.byref(cfg)), "kache_amd_get_tuning")
+        return {f: getattr(cfg, f) for f, _ in Tuning._fields_}
+
+    def autotune(self, rounds=2):
+        self.api.check(self.api.lib.kache_amd_autotune(self.h, rounds), "kache_amd_autotune")
+
+    def tuning_trials(self):
+        n = C.c_size_t()
+        self.api.check(self.api.lib.kache_amd_tuning_trials(self.h, None, 0, C.byref(n)), "kache_amd_tuning_trials")
+        out = (TuningTrial * max(n.value, 1))()
+        self.api.check(self.api.lib.kache_amd_tuning_trials(self.h, out, n.value, C.byref(n)), "kache_amd_tuning_trials")
+        return [{"blocks_per_cu": t.cfg.blocks_per_cu, "launch_log2": t.cfg.launch_log2, "graph": t.cfg.graph,
+                 "launches": t.launches, "median_ns_per_position": t.median_ns_per_position} for t in out[:n.value]]
 
     def set_blocks_per_cu(self, blocks):
         self.api.check(self.api.lib.kache_amd_set_blocks_per_cu(self.h, blocks), "kache_amd_set_blocks_per_cu")

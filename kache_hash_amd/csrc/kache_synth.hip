@@ -72,6 +72,46 @@ __host__ __device__ __forceinline__ uint8_t read_char(const ReadParams& p, uint6
     return ch;
 }
 
+// Character j (0 <= j < len) of sequence i of length len at text offset
+// `base`: as read_char, with per-sequence lengths; the position hashes use
+// the text offset, unique to each character.
+__host__ __device__ __forceinline__ uint8_t seq_char(const ReadParams& p, uint64_t i, uint64_t len, uint64_t base, uint64_t j)
+{
+    const uint64_t h0 = mix(p.seed, 4 * i), h1 = mix(p.seed, 4 * i + 1), h2 = mix(p.seed, 4 * i + 2);
+    const uint64_t start = h0 % (p.genome_len - len + 1);
+    const bool rev = h1 & 1;
+    const uint64_t w = rev ? len - 1 - j : j;
+    uint32_t code = genome_code(p.genome_seed, start + w);
+    const uint64_t u = mix(p.seed ^ 0x5DEECE66Dull, base + w);
+    if((u & 0xFFFFFF) < p.sub_th)
+        code = (code + 1 + static_cast<uint32_t>((u >> 24) % 3)) & 3;
+    if(rev)
+        code = 3 - code;
+    uint8_t ch = base_char(code);
+    const uint64_t v = mix(p.seed ^ 0xB5AD4ECEDA1CE2A9ull, base + j);
+    if((v & 0xFFFFFF) < p.n_th)
+        ch = 'N';
+    if((h2 & 0xFFFFFF) < p.lower_th)
+    {
+        const uint64_t lo = (h2 >> 24) % len, span = 1 + (h2 >> 40) % len;
+        if(j >= lo && j < lo + span)
+            ch |= 0x20;
+    }
+    return ch;
+}
+
+// One block per sequence (grid-stride): sequence i fills [off[i], off[i+1] - 1)
+// and '\n' ends it.
+__global__ void k_synth_seqs(uint8_t* out, const uint64_t* off, ReadParams p)
+{
+    for(uint64_t i = blockIdx.x; i < p.n; i += gridDim.x)
+    {
+        const uint64_t b = off[i], len = off[i + 1] - b - 1;
+        for(uint64_t j = threadIdx.x; j <= len; j += blockDim.x)
+            out[b + j] = j == len ? uint8_t('\n') : seq_char(p, i, len, b, j);
+    }
+}
+
 __global__ void k_synth_genome(uint8_t* out, uint64_t n, uint64_t seed)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -181,6 +221,22 @@ int kache_amd_synth_reads(char* out, uint64_t n_reads, uint64_t read_len, uint64
     const uint64_t total = n_reads * (read_len + 1);
     const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((total + 255) / 256, 65536));
     hipLaunchKernelGGL(k_synth_reads, dim3(g), dim3(256), 0, 0, reinterpret_cast<uint8_t*>(out), p);
+    hipError_t e = hipGetLastError();
+    if(e == hipSuccess)
+        e = hipDeviceSynchronize();
+    return e == hipSuccess ? KACHE_OK : fail(e);
+}
+
+int kache_amd_synth_seqs(char* out, const uint64_t* offsets, uint64_t n, uint64_t max_len, uint64_t genome_seed, uint64_t genome_len,
+                         uint64_t seed, double sub, double n_rate, double lower) noexcept
+{
+    if((n > 0 && (out == nullptr || offsets == nullptr)) || genome_len < max_len || max_len == 0)
+        return KACHE_EINVAL;
+    if(n == 0)
+        return KACHE_OK;
+    const ReadParams p = params(genome_seed, genome_len, n, 0, seed, sub, n_rate, lower);
+    const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>(n, 65536));
+    hipLaunchKernelGGL(k_synth_seqs, dim3(g), dim3(256), 0, 0, reinterpret_cast<uint8_t*>(out), offsets, p);
     hipError_t e = hipGetLastError();
     if(e == hipSuccess)
         e = hipDeviceSynchronize();

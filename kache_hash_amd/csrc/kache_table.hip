@@ -22,6 +22,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <chrono>
 #include <vector>
 
 using namespace kamd;
@@ -236,6 +237,7 @@ struct kache_table
     mutable Counters* d_cnt = nullptr;
     mutable Counters* h_cnt = nullptr;
     uint32_t blocks_per_cu = 0;
+    mutable std::map<const void*, int> resident;   // blocks per CU at full occupancy, per kernel
 
     // Profiling.
     bool profiling = false;
@@ -281,6 +283,7 @@ struct kache_table
     {
         if(stream)
             (void)hipStreamSynchronize(stream);
+        free_graphs();
         if(buckets)
             (void)hipFree(buckets), buckets = nullptr;
         if(ov)
@@ -345,11 +348,23 @@ struct kache_table
     template <typename Kern>
     uint32_t grid_for(Kern kern, uint64_t work_blocks) const
     {
+        return grid_for(reinterpret_cast<const void*>(kern), work_blocks);
+    }
+
+    uint32_t grid_for(const void* kern, uint64_t work_blocks) const
+    {
         int per_cu = static_cast<int>(blocks_per_cu);
         if(per_cu == 0)
         {
-            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TILE, 0) != hipSuccess || per_cu < 1)
-                per_cu = 4;
+            const auto it = resident.find(kern);
+            if(it != resident.end())
+                per_cu = it->second;
+            else
+            {
+                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TILE, 0) != hipSuccess || per_cu < 1)
+                    per_cu = 4;
+                resident[kern] = per_cu;
+            }
         }
         const uint64_t g = static_cast<uint64_t>(per_cu) * device_cus();
         return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, work_blocks)));
@@ -627,58 +642,246 @@ struct kache_table
 
     // --- Inserts ---
 
-    template <bool MAP, uint32_t OP, bool OR>
-    void launch_insert(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr, Counters* cnt) const
+    // The insert kernel of an operation.
+    const void* insert_kernel(uint32_t op, bool oriented) const
     {
-        const TableView T = view();
-        const uint64_t tiles = (e - (b & ~uint64_t(15)) + TILE - 1) / TILE;
-        hipLaunchKernelGGL((k_insert<MAP, OP, OR>), dim3(grid_for(k_insert<MAP, OP, OR>, tiles)), dim3(TILE), 0, stream, T, text, text_len, b, e, x, xr, cnt);
+        const auto f = [](auto kern) { return reinterpret_cast<const void*>(kern); };
+        if(!map)
+            return f(k_insert<false, OP_KEEP, false>);
+        if(oriented)
+            switch(op)
+            {
+            case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, true>);
+            case OP_ADD:    return f(k_insert<true, OP_ADD, true>);
+            case OP_OR:     return f(k_insert<true, OP_OR, true>);
+            default:        return f(k_insert<true, OP_MAX, true>);
+            }
+        switch(op)
+        {
+        case OP_KEEP:   return f(k_insert<true, OP_KEEP, false>);
+        case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, false>);
+        case OP_ADD:    return f(k_insert<true, OP_ADD, false>);
+        case OP_OR:     return f(k_insert<true, OP_OR, false>);
+        case OP_MIN:    return f(k_insert<true, OP_MIN, false>);
+        case OP_MAX:    return f(k_insert<true, OP_MAX, false>);
+        default:        return f(k_insert<true, OP_ADD_WRAP, false>);
+        }
+    }
+
+    // The arguments of one insert launch, in k_insert's order.
+    struct InsertArgs
+    {
+        TableView T;
+        const uint8_t* text;
+        uint64_t text_len, b, e, x, xr;
+        Counters* cnt;
+        void* ptrs[8];
+        void bind() { ptrs[0] = &T, ptrs[1] = &text, ptrs[2] = &text_len, ptrs[3] = &b, ptrs[4] = &e, ptrs[5] = &x, ptrs[6] = &xr, ptrs[7] = &cnt; }
+    };
+
+    InsertArgs insert_args(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr, Counters* cnt, bool oriented) const
+    {
+        InsertArgs a{view(), text, text_len, b, e, map ? x : 0, (map && oriented) ? xr : 0, cnt ? cnt : d_cnt, {}};
+        a.bind();
+        return a;
+    }
+
+    uint32_t insert_grid(const void* kern, uint64_t b, uint64_t e) const
+    {
+        return grid_for(kern, (e - (b & ~uint64_t(15)) + TILE - 1) / TILE);
     }
 
     void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr,
                          Counters* cnt = nullptr) const
     {
-        if(cnt == nullptr)
-            cnt = d_cnt;
-        if(!map)
-            return launch_insert<false, OP_KEEP, false>(text, text_len, b, e, 0, 0, cnt);
-        if(oriented)
-            switch(op)
-            {
-            case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, true>(text, text_len, b, e, x, xr, cnt);
-            case OP_ADD:    return launch_insert<true, OP_ADD, true>(text, text_len, b, e, x, xr, cnt);
-            case OP_OR:     return launch_insert<true, OP_OR, true>(text, text_len, b, e, x, xr, cnt);
-            default:        return launch_insert<true, OP_MAX, true>(text, text_len, b, e, x, xr, cnt);
-            }
-        switch(op)
+        const void* kern = insert_kernel(op, oriented);
+        InsertArgs a = insert_args(text, text_len, b, e, x, xr, cnt, oriented);
+        HIP(hipLaunchKernel(kern, dim3(insert_grid(kern, b, e)), dim3(TILE), a.ptrs, 0, stream));
+    }
+
+    // --- Launch shape and its online tuner ---
+
+    // Positions per insert launch are 2^launch_log2; blocks per CU as
+    // blocks_per_cu; with use_graph, a launch and its counter read-back run
+    // as one replay of a captured hipGraph whose kernel node takes the new
+    // arguments.
+    uint32_t launch_log2 = 24;
+    bool use_graph = false;
+
+    struct Trial
+    {
+        kache_amd_tuning cfg;
+        std::vector<double> ns_per_pos;
+    };
+    std::vector<Trial> trials;
+    uint32_t tune_rounds = 0;           // > 0 while tuning
+    kache_amd_tuning tuned_from{};
+    size_t tune_next = 0;
+
+    // One graph per insert kernel: [kernel, counters to host, zero counters].
+    struct InsertGraph
+    {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipGraphNode_t kernel = nullptr;
+    };
+    mutable std::map<const void*, InsertGraph> graphs;
+
+    void free_graphs() const
+    {
+        for(auto& [k, g] : graphs)
         {
-        case OP_KEEP:   return launch_insert<true, OP_KEEP, false>(text, text_len, b, e, x, 0, cnt);
-        case OP_ASSIGN: return launch_insert<true, OP_ASSIGN, false>(text, text_len, b, e, x, 0, cnt);
-        case OP_ADD:    return launch_insert<true, OP_ADD, false>(text, text_len, b, e, x, 0, cnt);
-        case OP_OR:     return launch_insert<true, OP_OR, false>(text, text_len, b, e, x, 0, cnt);
-        case OP_MIN:    return launch_insert<true, OP_MIN, false>(text, text_len, b, e, x, 0, cnt);
-        case OP_MAX:    return launch_insert<true, OP_MAX, false>(text, text_len, b, e, x, 0, cnt);
-        default:        return launch_insert<true, OP_ADD_WRAP, false>(text, text_len, b, e, x, 0, cnt);
+            if(g.exec)
+                (void)hipGraphExecDestroy(g.exec);
+            if(g.graph)
+                (void)hipGraphDestroy(g.graph);
         }
+        graphs.clear();
+    }
+
+    // Launches one insert and its counter read-back as a graph replay; the
+    // counters are then in h_cnt once the stream is synchronized.
+    void graph_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
+    {
+        const void* kern = insert_kernel(op, oriented);
+        InsertArgs a = insert_args(text, text_len, b, e, x, xr, nullptr, oriented);
+        const uint32_t grid = insert_grid(kern, b, e);
+        auto& g = graphs[kern];
+        if(!g.exec)
+        {
+            HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(TILE), a.ptrs, 0, stream);
+            if(le == hipSuccess)
+                le = hipMemcpyAsync(h_cnt, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream);
+            if(le == hipSuccess)
+            {
+                hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), sizeof(Counters) / 8);
+                le = hipGetLastError();
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(stream, &graph);
+            HIP(le);
+            HIP(ce);
+            g.graph = graph;
+            HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+            size_t n = 0;
+            HIP(hipGraphGetNodes(g.graph, nullptr, &n));
+            std::vector<hipGraphNode_t> nodes(n);
+            HIP(hipGraphGetNodes(g.graph, nodes.data(), &n));
+            for(auto node : nodes)
+            {
+                hipGraphNodeType ty;
+                HIP(hipGraphNodeGetType(node, &ty));
+                if(ty != hipGraphNodeTypeKernel)
+                    continue;
+                hipKernelNodeParams kp{};
+                HIP(hipGraphKernelNodeGetParams(node, &kp));
+                if(kp.func == kern)
+                    g.kernel = node;
+            }
+            if(!g.kernel)
+                throw hip_failure("the captured insert graph has no insert kernel node");
+        }
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(kern);
+        kp.gridDim = dim3(grid);
+        kp.blockDim = dim3(TILE);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = a.ptrs;
+        kp.extra = nullptr;
+        HIP(hipGraphExecKernelNodeSetParams(g.exec, g.kernel, &kp));
+        HIP(hipGraphLaunch(g.exec, stream));
+    }
+
+    // One insert launch of [b, e) and its counters, by the current launch
+    // shape; while tuning, the shape is the next candidate's and the launch
+    // is timed.
+    Counters insert_launch(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr)
+    {
+        Trial* trial = nullptr;
+        if(tune_rounds)
+        {
+            trial = &trials[tune_next % trials.size()];
+            blocks_per_cu = trial->cfg.blocks_per_cu, use_graph = trial->cfg.graph != 0;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        Counters c;
+        if(use_graph)
+        {
+            timed("insert", [&]{ graph_insert(op, oriented, text, text_len, b, e, x, xr); });
+            sync();
+            c = *h_cnt;
+        }
+        else
+        {
+            timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
+            c = read_counters();
+        }
+        if(trial)
+        {
+            trial->ns_per_pos.push_back(std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / double(e - b));
+            if(++tune_next == trials.size() * tune_rounds)
+                finish_tuning();
+        }
+        return c;
+    }
+
+    // Candidates of the tuner: blocks per CU x graph replay, each tried on
+    // `rounds` launches in turn; launch sizes are tried likewise.
+    void start_tuning(uint32_t rounds)
+    {
+        trials.clear();
+        tune_next = 0;
+        tune_rounds = rounds;
+        if(!rounds)
+            return;
+        tuned_from = kache_amd_tuning{blocks_per_cu, launch_log2, use_graph ? 1u : 0u, 0};
+        int resident = 0;
+        if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, insert_kernel(OP_ADD, false), TILE, 0) != hipSuccess || resident < 1)
+            resident = 8;
+        std::vector<uint32_t> blocks{0};
+        for(uint32_t b : {uint32_t(resident * 3 / 4), uint32_t(resident / 2)})
+            if(b >= 1 && b != static_cast<uint32_t>(resident) && std::find(blocks.begin(), blocks.end(), b) == blocks.end())
+                blocks.push_back(b);
+        for(uint32_t lg : {23u, 25u})
+            for(uint32_t b : blocks)
+                for(uint32_t g : {0u, 1u})
+                    trials.push_back({kache_amd_tuning{b, lg, g, 0}, {}});
+    }
+
+    void finish_tuning()
+    {
+        double best = 0;
+        for(const auto& t : trials)
+        {
+            std::vector<double> v = t.ns_per_pos;
+            std::sort(v.begin(), v.end());
+            const double med = v.empty() ? 0 : v[v.size() / 2];
+            if(!v.empty() && (best == 0 || med < best))
+                best = med, blocks_per_cu = t.cfg.blocks_per_cu, use_graph = t.cfg.graph != 0, launch_log2 = t.cfg.launch_log2;
+        }
+        tune_rounds = 0;
     }
 
     // Inserts the positions of a device text, in launches of at most
-    // LAUNCH_POS positions, growing the overflow table ahead of each so it
+    // 2^launch_log2 positions, growing the overflow table ahead of each so it
     // cannot fill, and the table after each as the schedule says.
     uint64_t insert_text(CallSched& s, uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t x, uint64_t xr,
                          uint64_t first = 0)
     {
         uint64_t inserted = 0;
-        for(uint64_t b = first; b < text_len; b = (b & ~(LAUNCH_POS - 1)) + LAUNCH_POS)
+        for(uint64_t b = first; b < text_len;)
         {
-            const uint64_t e = std::min(text_len, (b & ~(LAUNCH_POS - 1)) + LAUNCH_POS);
+            if(tune_rounds)
+                launch_log2 = trials[tune_next % trials.size()].cfg.launch_log2;
+            const uint64_t e = std::min(text_len, b + (uint64_t(1) << launch_log2));
             ov_room(e - b);
-            timed("insert", [&]{ dispatch_insert(op, oriented, text, text_len, b, e, x, xr); });
-            const Counters c = read_counters();
+            const Counters c = insert_launch(op, oriented, text, text_len, b, e, x, xr);
             if(c.failed)
                 throw hip_failure("an insert found no room in the overflow table");
             inserted += c.inserted, size += c.inserted, ov_count += c.ov_inserted;
             grow(sched_add(s, c.inserted));
+            b = e;
         }
         return inserted;
     }
@@ -1787,6 +1990,48 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) noexcept
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
     t->blocks_per_cu = blocks;
+    return KACHE_OK;
+}
+
+int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) noexcept
+{
+    if(t == nullptr || cfg == nullptr || cfg->blocks_per_cu > 64 || cfg->launch_log2 < 16 || cfg->launch_log2 > 30 || cfg->graph > 1)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    t->blocks_per_cu = cfg->blocks_per_cu, t->launch_log2 = cfg->launch_log2, t->use_graph = cfg->graph != 0;
+    t->tune_rounds = 0;
+    return KACHE_OK;
+}
+
+int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) noexcept
+{
+    if(t == nullptr || cfg == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    *cfg = kache_amd_tuning{t->blocks_per_cu, t->launch_log2, t->use_graph ? 1u : 0u, t->tune_rounds};
+    return KACHE_OK;
+}
+
+int kache_amd_autotune(kache_table* t, uint32_t rounds) noexcept
+{
+    if(t == nullptr || rounds > 16)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    return guard([&](){ t->start_tuning(rounds); return static_cast<int>(KACHE_OK); });
+}
+
+int kache_amd_tuning_trials(const kache_table* t, kache_amd_tuning_trial* out, size_t cap, size_t* n) noexcept
+{
+    if(t == nullptr || n == nullptr || (cap > 0 && out == nullptr))
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    *n = t->trials.size();
+    for(size_t i = 0; i < std::min(cap, t->trials.size()); ++i)
+    {
+        std::vector<double> v = t->trials[i].ns_per_pos;
+        std::sort(v.begin(), v.end());
+        out[i] = kache_amd_tuning_trial{t->trials[i].cfg, static_cast<uint32_t>(v.size()), v.empty() ? 0.0 : v[v.size() / 2]};
+    }
     return KACHE_OK;
 }
 
