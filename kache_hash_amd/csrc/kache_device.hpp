@@ -368,6 +368,206 @@ __device__ __forceinline__ Res insert_key(const TableView& T, uint64_t key, uint
     return ov_insert<MAP>(T, key, sk, slot);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative inserts.  Slot words hold 0 (empty), a stored key
+// (~key: top bits 11), a reservation (stored key with bit 63 flipped: top
+// bits 01, the key still readable) or a tombstone (TOMB: top bits 10).
+// Reservations and tombstones exist only inside insert launches; tombstones
+// carry a nonzero checksum, never match a key, and vanish at the next
+// doubling.
+
+constexpr uint64_t RESERVE_FLIP = uint64_t(1) << 63;
+constexpr uint64_t TOMB = uint64_t(1) << 63;
+constexpr uint32_t CLAIM_ROUNDS = 256;
+
+__device__ __forceinline__ bool is_key(uint64_t v) { return (v >> 62) == 3; }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
+{
+    return (uint64_t(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), l))) << 32) |
+           static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
+}
+
+__device__ __forceinline__ uint64_t exchange64(uint64_t* p, uint64_t v)
+{
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inserts with the policy of insert_key, for all lanes of a wave at once
+// (every lane calls it; `active` lanes have a key).  Lanes that need a new
+// slot in the same bucket, from the same lowest possibly-free slot z, take
+// consecutive slots z, z + 1, ... in one round of CASes, ranked by lane,
+// instead of contending for one slot at a time: the first claims with its
+// key (nothing below z can hold it: those slots' checksums were read), the
+// others with a reservation.  A reserving lane then reads the slots from z
+// below its own: all are taken (by the round's other lanes or by others
+// before), so the key can only be there as a key or a reservation; if it is,
+// the lowest such slot is the key's (its owner sees no copy below it) and the
+// lane turns its reservation into a tombstone; otherwise it writes its key
+// over the reservation.  Nothing waits on anything, so a wave never blocks
+// on another.  A lane whose CAS loses learns that the slot is taken and
+// tries again above it.
+template <bool MAP>
+__device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
+{
+    const uint64_t sk = ~key;
+    in_ov = false;
+    slot = nullptr;
+    Res r = RES_FOUND;
+    // Per-lane state: the policy's stage (0 primary, 1 p, 2 q) and the lowest
+    // slot of the current bucket not known to be taken.
+    uint32_t stage = 0, lo = 0;
+    uint64_t bidx = active ? bucket_of(h, T.cap) : 0;
+    bool need = active;
+    for(uint32_t round = 0; __ballot(need); ++round)
+    {
+        if(round == CLAIM_ROUNDS)
+        {
+            if(need)
+                r = RES_FULL, need = false;
+            break;
+        }
+        uint8_t* bk = bucket_at<MAP>(T, bidx);
+        uint32_t z = B;
+        if(need)
+        {
+            uint32_t eq, zero;
+            cs_masks(bk, c, eq, zero);
+            while(eq)
+            {
+                const uint32_t j = __builtin_ctz(eq);
+                eq &= eq - 1;
+                uint64_t* s = slot_at<MAP>(bk, j);
+                if(ld_acq(s) == sk)
+                {
+                    slot = s, r = RES_FOUND, need = false;
+                    break;
+                }
+            }
+            z = zero ? __builtin_ctz(zero) : B;
+            z = z > lo ? z : lo;
+        }
+        const uint64_t claimers = __ballot(need);
+        if(!claimers)
+            break;
+        // Groups: the wave's claimers of the same bucket from the same slot.
+        const uint64_t gkey = (bidx << 6) | z;
+        uint64_t grp = 0;
+        for(uint64_t rem = claimers; rem;)
+        {
+            const uint64_t gl = readlane64(gkey, __builtin_ctzll(rem));
+            const uint64_t g = __ballot(need && gkey == gl);
+            if(need && gkey == gl)
+                grp = g;
+            rem &= ~g;
+        }
+        const uint64_t below_me = (uint64_t(1) << __lane_id()) - 1;
+        // A lane whose key a lower lane of its group also has follows that lane.
+        int owner = -1;
+        if(need)
+            for(uint64_t m = grp & below_me; m; m &= m - 1)
+            {
+                const int l = __builtin_ctzll(m);
+                if(static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(sk), l, 64)) == sk)
+                {
+                    owner = l;
+                    break;
+                }
+            }
+        const uint64_t leads = __ballot(need && owner < 0);
+        if(need && owner < 0)
+        {
+            const uint64_t rk = sk ^ RESERVE_FLIP;
+            const uint32_t rank = __popcll(grp & leads & below_me);
+            const uint32_t j = z + rank;
+            // The first of a group has no slot to check below its own (those
+            // below z hold keys whose checksums were read): it claims with its
+            // key at once.  The others reserve, then check.
+            bool mine = false;
+            uint64_t occ = 0;
+            if(j < B)
+                mine = cas64(slot_at<MAP>(bk, j), occ, rank == 0 ? sk : rk);
+            if(mine && rank == 0)
+            {
+                bk[j] = c;
+                slot = slot_at<MAP>(bk, j), r = RES_NEW, need = false;
+            }
+            else
+            {
+                // The lowest slot from z that holds or will hold this key.
+                uint32_t cand = B;
+                const uint32_t top = j < B ? j : B;
+                for(uint32_t t = z; t < top; ++t)
+                {
+                    const uint64_t v = ld_acq(slot_at<MAP>(bk, t));
+                    if(v == sk || v == rk)
+                    {
+                        cand = t;
+                        break;
+                    }
+                }
+                if(cand == B && j < B && !mine && (occ == sk || occ == rk))
+                    cand = j;
+                if(cand < B)
+                {
+                    if(mine && exchange64(slot_at<MAP>(bk, j), TOMB) == rk)
+                        bk[j] = c;
+                    slot = slot_at<MAP>(bk, cand), r = RES_FOUND, need = false;
+                }
+                else if(mine)
+                {
+                    if(exchange64(slot_at<MAP>(bk, j), sk) == rk)
+                        bk[j] = c;
+                    slot = slot_at<MAP>(bk, j), r = RES_NEW, need = false;
+                }
+                else if(j >= B)
+                {
+                    // Every slot is taken and none holds the key: the next bucket of the policy.
+                    if(stage == 0)
+                        mark_spill(bk, h);
+                    if(stage < 2)
+                    {
+                        uint64_t p, q;
+                        secondaries(h, T.cap, p, q);
+                        bidx = stage == 0 ? p : q;
+                        stage++, lo = 0;
+                    }
+                    else
+                    {
+                        in_ov = true;
+                        r = ov_insert<MAP>(T, ~sk, sk, slot);
+                        need = false;
+                    }
+                }
+                else
+                    lo = j + 1;     // taken since the checksums were read: slots up to j are known taken
+            }
+        }
+        if(__ballot(owner >= 0))
+        {
+            // Followers take their owner's outcome, or its next step.
+            const int src = owner >= 0 ? owner : static_cast<int>(__lane_id());
+            const uint32_t o_state = __shfl(static_cast<int>(need) | (static_cast<int>(r) << 1) | (static_cast<int>(in_ov) << 3) |
+                                            static_cast<int>(stage << 4) | static_cast<int>(lo << 8), src, 64);
+            const uint64_t o_slot = __shfl(static_cast<unsigned long long>(reinterpret_cast<uintptr_t>(slot)), src, 64);
+            const uint64_t o_b = __shfl(static_cast<unsigned long long>(bidx), src, 64);
+            if(owner >= 0)
+            {
+                if(!(o_state & 1))
+                {
+                    slot = reinterpret_cast<uint64_t*>(static_cast<uintptr_t>(o_slot));
+                    r = ((o_state >> 1) & 3) == RES_FULL ? RES_FULL : RES_FOUND;
+                    in_ov = (o_state >> 3) & 1;
+                    need = false;
+                }
+                else
+                    bidx = o_b, stage = (o_state >> 4) & 15, lo = o_state >> 8;
+            }
+        }
+    }
+    return r;
+}
+
 // Lookup policy (include/Streaming_Kmer_Hash_Table.hpp:2548-2574): the
 // primary; absent unless it is full and its filter marks the minimizer; the
 // secondaries; the overflow table iff both are full.

@@ -39,11 +39,11 @@ __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restri
     uint64_t ins = 0, ins_ov = 0, failed = 0;
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
-        if(!d.valid)
-            return;
         uint64_t* slot;
         bool in_ov;
-        const Res r = insert_key<MAP>(T, d.key, d.h, d.c, slot, in_ov);
+        const Res r = insert_key_wave<MAP>(T, d.valid, d.key, d.h, d.c, slot, in_ov);
+        if(!d.valid)
+            return;
         if(r == RES_FULL)
             failed++;
         else
@@ -207,9 +207,11 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
         {
             uint8_t* ob = bucket_at<MAP>(O, b);
             c = ob[j];
+            const uint64_t* os = slot_at<MAP>(ob, j);
+            if(c && !is_key(os[0]))
+                c = 0;      // a tombstone: dropped
             if(c)
             {
-                const uint64_t* os = slot_at<MAP>(ob, j);
                 sk = os[0];
                 if(MAP)
                     val = os[1];
@@ -289,33 +291,101 @@ __device__ __forceinline__ void reinsert_one(const TableView& N, uint64_t sk, ui
 }
 
 // Phase 2: the keys marked away from their primary buckets, then the old
-// overflow table's, go in with the insertion policy (:1575-1665).  A lane
-// takes one word of marks (two old buckets) and walks its set bits, so every
-// lane has work; then one overflow slot.
+// overflow table's, go in with the insertion policy (:1575-1665).  A wave
+// takes 64 words of marks (128 old buckets), numbers their marked keys by a
+// prefix sum over the lanes, and inserts them 64 at a time, one per lane,
+// with the wave-cooperative claims; then 64 old overflow slots at a time.
 template <bool MAP>
 __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableView O, TableView N, Counters* cnt)
 {
     __shared__ uint64_t red[TILE / 64];
     uint64_t ov_n = 0, failed = 0;
-    const uint64_t pairs = (O.cap + 1) / 2, all = pairs + O.ov_mask + 1;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t pairs = (O.cap + 1) / 2, mark_chunks = (pairs + 63) / 64;
+    const uint64_t ov_chunks = (O.ov_mask + 1 + 63) / 64, chunks = mark_chunks + ov_chunks;
+    const uint64_t waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    for(uint64_t ch = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; ch < chunks; ch += waves)
     {
-        if(g < pairs)
+        if(ch < mark_chunks)
         {
-            for(uint64_t m = away[g]; m; m &= m - 1)
+            const uint64_t w = ch * 64 + lane;
+            const uint64_t word = w < pairs ? away[w] : 0;
+            const uint32_t n = __popcll(word);
+            uint32_t incl = n;
+            for(uint32_t o = 1; o < 64; o <<= 1)
             {
-                const uint32_t bit = __builtin_ctzll(m);
-                const uint64_t* os = slot_at<MAP>(bucket_at<MAP>(O, 2 * g + (bit >> 5)), bit & 31);
-                reinsert_one<MAP>(N, os[0], MAP ? os[1] : 0, ov_n, failed);
+                const uint32_t t = __shfl_up(incl, o, 64);
+                if(lane >= o)
+                    incl += t;
+            }
+            const uint32_t total = __shfl(incl, 63, 64);
+            for(uint32_t base = 0; base < total; base += 64)
+            {
+                const uint32_t k = base + lane;
+                const bool active = k < total;
+                // The lane holding key k: the first whose inclusive count exceeds k.
+                uint32_t src = 0;
+                for(uint32_t step = 32; step; step >>= 1)
+                    if(__shfl(incl, static_cast<int>(src + step - 1), 64) <= k)
+                        src += step;
+                src = active ? src : 0;
+                uint64_t m = __shfl(static_cast<unsigned long long>(word), static_cast<int>(src), 64);
+                const uint32_t before = __shfl(incl, static_cast<int>(src), 64) - __shfl(n, static_cast<int>(src), 64);
+                uint64_t sk = 0, val = 0;
+                if(active)
+                {
+                    for(uint32_t i = k - before; i; --i)
+                        m &= m - 1;
+                    const uint32_t bit = __builtin_ctzll(m);
+                    const uint64_t* os = slot_at<MAP>(bucket_at<MAP>(O, 2 * (ch * 64 + src) + (bit >> 5)), bit & 31);
+                    sk = os[0];
+                    if(MAP)
+                        val = os[1];
+                }
+                uint64_t* slot;
+                bool in_ov;
+                const uint64_t key = ~sk;
+                const Res r = insert_key_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+                if(active)
+                {
+                    if(r == RES_FULL)
+                        failed++;
+                    else
+                    {
+                        ov_n += in_ov;
+                        if(MAP)
+                            st_rel(slot + 1, val);
+                    }
+                }
             }
         }
         else
         {
-            const uint64_t* os = O.ov + (g - pairs) * Geo<MAP>::slot_words;
-            const uint64_t sk = os[0];
-            if(sk)
-                reinsert_one<MAP>(N, sk, MAP ? os[1] : 0, ov_n, failed);
+            const uint64_t i = (ch - mark_chunks) * 64 + lane;
+            uint64_t sk = 0, val = 0;
+            if(i <= O.ov_mask)
+            {
+                const uint64_t* os = O.ov + i * Geo<MAP>::slot_words;
+                sk = os[0];
+                if(MAP && sk)
+                    val = os[1];
+            }
+            const bool active = sk != 0;
+            uint64_t* slot;
+            bool in_ov;
+            const uint64_t key = ~sk;
+            const Res r = insert_key_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+            if(active)
+            {
+                if(r == RES_FULL)
+                    failed++;
+                else
+                {
+                    ov_n += in_ov;
+                    if(MAP)
+                        st_rel(slot + 1, val);
+                }
+            }
         }
     }
     block_add(ov_n, &cnt->ov_inserted, red);
@@ -367,7 +437,9 @@ __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint
             {
                 const uint64_t* s = slot_at<MAP>(bk, g % B);
                 sk = s[0];
-                if(MAP)
+                if(!is_key(sk))
+                    sk = 0;
+                else if(MAP)
                     val = s[1];
             }
         }
@@ -414,6 +486,8 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
             if(!bk[g % B])
                 continue;
             s = slot_at<MAP>(bk, g % B);
+            if(!is_key(s[0]))
+                continue;
         }
         else
         {
@@ -457,6 +531,8 @@ __global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, u
             if(!bk[g % B])
                 continue;
             s = slot_at<true>(bk, g % B);
+            if(!is_key(s[0]))
+                continue;
         }
         else
         {
@@ -492,7 +568,7 @@ __global__ __launch_bounds__(TILE) void k_placement(TableView T, uint64_t* out)
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
     {
         uint8_t* bk = bucket_at<MAP>(T, g / B);
-        if(!bk[g % B])
+        if(!bk[g % B] || !is_key(slot_at<MAP>(bk, g % B)[0]))
             continue;
         const uint64_t h = minimizer_hash(~slot_at<MAP>(bk, g % B)[0]);
         if(bucket_of(h, T.cap) == g / B)
