@@ -11,9 +11,9 @@
  * sequences in device memory, each sequence followed by at least one
  * non-ACGT byte, so that no k-mer spans two of them.  The allocation must be
  * 16-byte aligned and readable for KACHE_AMD_TEXT_PAD bytes past text_bytes;
- * those bytes never count as bases.  The calls order themselves after all
- * prior work of the device (they synchronize the device before starting), and
- * return after their own work is done.
+ * those bytes never count as bases.  The text's producer must have finished
+ * before the call (a synchronous copy, or a synchronized stream): the calls
+ * run on the table's own stream, and return after their work is done.
  */
 
 #include "kache_capi.h"
@@ -48,6 +48,16 @@ int kache_amd_count_batches(const kache_table* t, const char* text, const uint64
  * arrays) or any of their entries may be NULL. */
 int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, const uint64_t* group_first,
                             size_t n_groups, uint8_t* const* found, void* const* values, uint8_t* const* fwd) KACHE_NOEXCEPT;
+
+/* kache_count and kache_amd_lookup_groups over n sequences already in a
+ * device text: sequence i spans [offsets[i], offsets[i + 1] - 1) and a
+ * non-ACGT byte ends it at offsets[i + 1] - 1 (offsets: n + 1 host
+ * entries, offsets[n] <= text_bytes).  Outputs as the host-sequence calls. */
+int kache_amd_count_seqs_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
+                                uint64_t* counts) KACHE_NOEXCEPT;
+int kache_amd_lookup_groups_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
+                                   const uint64_t* group_first, size_t n_groups, uint8_t* const* found, void* const* values,
+                                   uint8_t* const* fwd) KACHE_NOEXCEPT;
 
 /* The number of k-mer positions of a device text present in the table
  * (`hits`) and, unless NULL, of all its k-mer positions (`positions`). */
@@ -186,18 +196,21 @@ int kache_amd_synth_seqs(char* out, const uint64_t* offsets, uint64_t n, uint64_
 const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
 
 /* Serving loop over a resident table.  Requests are submitted from any
- * thread and run by one worker thread of the server, in waves: the worker
- * takes the pending requests in arrival order, as many as fit `wave_bytes`
- * of staged sequence (a larger request runs as a wave of its own) and at
- * most `max_wave_requests`, and runs the wave in three phases: all its reads
- * (COUNT and LOOKUP) merged into one call each, then its updates one by one,
- * then its upserts, one pipelined batch call (kache_amd_insert_batches) per
- * run of equal op and operand, a batch per request.  Requests arriving while
- * a wave runs join the next one; a request completes when its phase ends.
- * Each request's results are those of
- * running all requests alone, one after another, in served order: by wave,
- * then phase (reads 0, updates 1, upserts 2), then ticket.  With
- * max_wave_requests = 1 every request is one plain C-ABI call.
+ * thread and served in waves: pending requests are taken in arrival order,
+ * as many as fit `wave_bytes` of staged sequence (a larger request is a wave
+ * of its own) and at most `max_wave_requests`.  Two threads pipeline the
+ * waves: a stager packs wave k + 1's sequences into one of two pinned
+ * buffers and copies them to the device (its HBM budget: two such texts)
+ * while an executor runs wave k from the other; a wave is admitted when a
+ * buffer is free, and requests arriving meanwhile join it.  A wave runs in
+ * three phases: its reads (COUNT and LOOKUP, each kind merged into one
+ * device call), its updates, then its upserts (one pipelined batch call per
+ * run of equal op and operand, one batch per request); a request completes
+ * when its phase ends.  Each request's results are those of running all
+ * requests alone, one after another, in served order: by wave, then phase
+ * (reads 0, updates 1, upserts 2), then ticket.  With max_wave_requests = 1
+ * there is no pipeline: every request is one plain C-ABI call on one
+ * worker thread (the per-request baseline).
  *
  * The request's arrays (sequences, outputs) must stay valid until its wait
  * returns.  Upserts run as `user` (from kache_register); while a server

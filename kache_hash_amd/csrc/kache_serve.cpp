@@ -70,6 +70,21 @@ uint32_t phase_of(uint32_t kind)
 
 }
 
+// A wave staged for the device: its requests by phase, and where their
+// sequences sit in the slot's text (each sequence followed by '\n').
+struct Staged
+{
+    uint64_t id = 0;
+    int slot = 0;
+    std::vector<std::shared_ptr<Req>> reqs;
+    std::vector<Req*> counts, lookups, updates, inserts;
+    uint64_t count_at = 0, lookup_at = 0, insert_at = 0;       // region starts (16-byte aligned)
+    std::vector<uint64_t> count_off, lookup_off;               // sequence offsets within their region
+    std::vector<uint64_t> update_at, update_bytes;             // per update request
+    std::vector<uint64_t> insert_bounds;                       // per insert request, within its region
+    uint64_t bytes = 0;                                        // staged text bytes
+};
+
 struct kache_amd_server
 {
     kache_table* t;
@@ -79,24 +94,41 @@ struct kache_amd_server
     uint32_t width;
 
     std::mutex mu;
-    std::condition_variable cv_in, cv_done;
+    std::condition_variable cv_in, cv_done, cv_staged, cv_slot;
     std::deque<std::shared_ptr<Req>> pending;
     std::map<uint64_t, std::shared_ptr<Req>> live;
     uint64_t next_ticket = 1;
     bool stopping = false;
-    std::thread worker;
+    std::thread worker, stager;
 
-    // Staging of a wave's upserts: pinned host bytes, their device copy.
+    // Plain path: staging of a wave's upserts.
     char* h_text = nullptr;
     void* d_text = nullptr;
     uint64_t h_bytes = 0, d_bytes = 0;
+
+    // Pipelined path: two slots, each a pinned host text and its device copy.
+    // The stager packs and copies wave k + 1 into one while the executor
+    // runs wave k from the other.
+    static constexpr int SLOTS = 2;
+    char* h_slot[SLOTS] = {};
+    void* d_slot[SLOTS] = {};
+    uint64_t slot_bytes[SLOTS] = {};
+    bool slot_busy[SLOTS] = {};
+    std::deque<std::unique_ptr<Staged>> staged;
+    bool stager_done = false;
 
     kache_amd_server_stats stats{};
 
     kache_amd_server(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_req, uint32_t width)
         : t(t), user(user), wave_bytes(wave_bytes), max_wave_requests(max_req), width(width)
     {
-        worker = std::thread([this]{ loop(); });
+        if(max_wave_requests == 1)
+            worker = std::thread([this]{ loop(); });
+        else
+        {
+            stager = std::thread([this]{ stage_loop(); });
+            worker = std::thread([this]{ exec_loop(); });
+        }
     }
 
     ~kache_amd_server()
@@ -106,12 +138,44 @@ struct kache_amd_server
             stopping = true;
         }
         cv_in.notify_all();
+        cv_slot.notify_all();
+        if(stager.joinable())
+            stager.join();
         worker.join();
         if(d_text)
             kache_amd_device_free(d_text);
         if(h_text)
             kache_amd_host_free(h_text);
+        for(int i = 0; i < SLOTS; ++i)
+        {
+            if(d_slot[i])
+                kache_amd_device_free(d_slot[i]);
+            if(h_slot[i])
+                kache_amd_host_free(h_slot[i]);
+        }
     }
+
+    // Takes the pending requests of the next wave (the lock held): in
+    // arrival order, as many as fit wave_bytes (at least one) and
+    // max_wave_requests.
+    std::vector<std::shared_ptr<Req>> take_wave(uint64_t& wave_id)
+    {
+        std::vector<std::shared_ptr<Req>> wave;
+        uint64_t bytes = 0;
+        while(!pending.empty() && wave.size() < max_wave_requests && (wave.empty() || bytes + pending.front()->bytes <= wave_bytes))
+        {
+            bytes += pending.front()->bytes;
+            wave.push_back(pending.front());
+            pending.pop_front();
+        }
+        wave_id = ++stats.waves;
+        stats.requests += wave.size();
+        stats.max_wave_requests = std::max<uint64_t>(stats.max_wave_requests, wave.size());
+        stats.max_wave_bytes = std::max(stats.max_wave_bytes, bytes);
+        return wave;
+    }
+
+    // --- Plain path: one worker, one request per wave, plain C-ABI calls ---
 
     void loop()
     {
@@ -124,18 +188,7 @@ struct kache_amd_server
                 cv_in.wait(lock, [&]{ return stopping || !pending.empty(); });
                 if(pending.empty())
                     return;
-                uint64_t bytes = 0;
-                while(!pending.empty() && wave.size() < max_wave_requests &&
-                      (wave.empty() || bytes + pending.front()->bytes <= wave_bytes))
-                {
-                    bytes += pending.front()->bytes;
-                    wave.push_back(pending.front());
-                    pending.pop_front();
-                }
-                wave_id = ++stats.waves;
-                stats.requests += wave.size();
-                stats.max_wave_requests = std::max<uint64_t>(stats.max_wave_requests, wave.size());
-                stats.max_wave_bytes = std::max(stats.max_wave_bytes, bytes);
+                wave = take_wave(wave_id);
             }
             const double t0 = now_s();
             for(auto& q : wave)
@@ -310,6 +363,221 @@ struct kache_amd_server
             if(st == KACHE_OK && ins[i]->r.counts)
                 *ins[i]->r.counts = out[i - a];
         }
+    }
+
+    // --- Pipelined path ---
+
+    static uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t(15); }
+
+    // Packs a wave into slot `w.slot`'s pinned text and copies it to the device.
+    int pack(Staged& w)
+    {
+        for(auto& q : w.reqs)
+            switch(q->r.kind)
+            {
+            case KACHE_AMD_REQ_COUNT: w.counts.push_back(q.get()); break;
+            case KACHE_AMD_REQ_LOOKUP: w.lookups.push_back(q.get()); break;
+            case KACHE_AMD_REQ_UPDATE: w.updates.push_back(q.get()); break;
+            default: w.inserts.push_back(q.get()); break;
+            }
+        // Layout: counts, lookups, each update request, inserts; every region 16-byte aligned.
+        uint64_t o = 0;
+        const auto region = [&](const std::vector<Req*>& rs)
+        {
+            uint64_t b = 0;
+            for(Req* q : rs)
+                b += q->bytes;
+            const uint64_t at = o;
+            o = align16(o + b);
+            return at;
+        };
+        w.count_at = region(w.counts);
+        w.lookup_at = region(w.lookups);
+        for(Req* q : w.updates)
+            w.update_at.push_back(o), w.update_bytes.push_back(q->bytes), o = align16(o + q->bytes);
+        w.insert_at = region(w.inserts);
+        w.bytes = o;
+        const uint64_t need = o + KACHE_AMD_TEXT_PAD;
+        const int sl = w.slot;
+        if(need > slot_bytes[sl])
+        {
+            if(h_slot[sl])
+                kache_amd_host_free(h_slot[sl]), h_slot[sl] = nullptr;
+            if(d_slot[sl])
+                kache_amd_device_free(d_slot[sl]), d_slot[sl] = nullptr;
+            slot_bytes[sl] = 0;
+            const uint64_t want = std::max<uint64_t>(need, wave_bytes + 4096 + KACHE_AMD_TEXT_PAD);
+            void* hp = nullptr;
+            int st = kache_amd_host_alloc(want, &hp);
+            if(st != KACHE_OK)
+                return st;
+            h_slot[sl] = static_cast<char*>(hp);
+            st = kache_amd_device_alloc(want, &d_slot[sl]);
+            if(st != KACHE_OK)
+                return st;
+            slot_bytes[sl] = want;
+        }
+        char* h = h_slot[sl];
+        std::memset(h, '\n', need);
+        const auto put = [&](const kache_amd_request& r, uint64_t at, std::vector<uint64_t>* offs)
+        {
+            for(size_t j = 0; j < r.n; ++j)
+            {
+                if(offs)
+                    offs->push_back(at);
+                if(r.lens[j])
+                    std::memcpy(h + at, r.seqs[j], r.lens[j]);
+                at += r.lens[j] + 1;
+            }
+            return at;
+        };
+        uint64_t at = w.count_at;
+        for(Req* q : w.counts)
+            at = put(q->r, at, &w.count_off);
+        w.count_off.push_back(at);
+        for(auto& x : w.count_off)
+            x -= w.count_at;
+        at = w.lookup_at;
+        for(Req* q : w.lookups)
+            at = put(q->r, at, &w.lookup_off);
+        w.lookup_off.push_back(at);
+        for(auto& x : w.lookup_off)
+            x -= w.lookup_at;
+        for(size_t i = 0; i < w.updates.size(); ++i)
+            put(w.updates[i]->r, w.update_at[i], nullptr);
+        at = w.insert_at;
+        w.insert_bounds.push_back(0);
+        for(Req* q : w.inserts)
+        {
+            at = put(q->r, at, nullptr);
+            w.insert_bounds.push_back(at - w.insert_at);
+        }
+        return kache_amd_copy(d_slot[sl], h, need, 1);
+    }
+
+    void stage_loop()
+    {
+        while(true)
+        {
+            auto w = std::make_unique<Staged>();
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_in.wait(lock, [&]{ return stopping || !pending.empty(); });
+                if(pending.empty())
+                {
+                    stager_done = true;
+                    cv_staged.notify_all();
+                    return;
+                }
+                // A wave is admitted when a slot is free: requests arriving
+                // meanwhile wait, and join it.
+                cv_slot.wait(lock, [&]{ return !slot_busy[0] || !slot_busy[1]; });
+                w->slot = slot_busy[0] ? 1 : 0;
+                slot_busy[w->slot] = true;
+                w->reqs = take_wave(w->id);
+            }
+            const int st = pack(*w);
+            if(st != KACHE_OK)
+            {
+                for(auto& q : w->reqs)
+                    q->status = st;
+                std::vector<Req*> all;
+                for(auto& q : w->reqs)
+                    all.push_back(q.get());
+                finish(all);
+                std::lock_guard<std::mutex> lock(mu);
+                slot_busy[w->slot] = false;
+                cv_slot.notify_all();
+                continue;
+            }
+            std::lock_guard<std::mutex> lock(mu);
+            staged.push_back(std::move(w));
+            cv_staged.notify_all();
+        }
+    }
+
+    void exec_loop()
+    {
+        while(true)
+        {
+            std::unique_ptr<Staged> w;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_staged.wait(lock, [&]{ return !staged.empty() || stager_done; });
+                if(staged.empty())
+                    return;
+                w = std::move(staged.front());
+                staged.pop_front();
+            }
+            execute(*w);
+            std::lock_guard<std::mutex> lock(mu);
+            slot_busy[w->slot] = false;
+            cv_slot.notify_all();
+        }
+    }
+
+    void execute(Staged& w)
+    {
+        const double t0 = now_s();
+        for(auto& q : w.reqs)
+            q->t_start = t0, q->wave = w.id, q->phase = phase_of(q->r.kind);
+        const char* d = static_cast<const char*>(d_slot[w.slot]);
+
+        if(!w.counts.empty())
+        {
+            const size_t n = w.count_off.size() - 1;
+            std::vector<uint64_t> out(std::max<size_t>(n, 1));
+            const int st = kache_amd_count_seqs_device(t, d + w.count_at, w.count_off.back(), w.count_off.data(), n, out.data());
+            size_t o = 0;
+            for(Req* q : w.counts)
+            {
+                q->status = st;
+                if(st == KACHE_OK && q->r.n)
+                    std::memcpy(q->r.counts, out.data() + o, q->r.n * sizeof(uint64_t));
+                o += q->r.n;
+            }
+            finish(w.counts);
+        }
+        if(!w.lookups.empty())
+        {
+            std::vector<uint64_t> first;
+            std::vector<uint8_t*> found, fwd;
+            std::vector<void*> values;
+            uint64_t o = 0;
+            for(Req* q : w.lookups)
+            {
+                first.push_back(o), o += q->r.n;
+                found.push_back(q->r.found), values.push_back(q->r.values), fwd.push_back(q->r.fwd);
+            }
+            const int st = kache_amd_lookup_groups_device(t, d + w.lookup_at, w.lookup_off.back(), w.lookup_off.data(), w.lookup_off.size() - 1,
+                                                          first.data(), w.lookups.size(), found.data(), values.data(), fwd.data());
+            for(Req* q : w.lookups)
+                q->status = st;
+            finish(w.lookups);
+        }
+        for(size_t i = 0; i < w.updates.size(); ++i)
+        {
+            Req* q = w.updates[i];
+            q->status = kache_amd_update_device(t, d + w.update_at[i], w.update_bytes[i], q->r.op, q->r.x, q->r.counts);
+        }
+        finish(w.updates);
+        for(size_t a = 0; a < w.inserts.size();)
+        {
+            size_t b = a + 1;
+            while(b < w.inserts.size() && w.inserts[b]->r.op == w.inserts[a]->r.op && w.inserts[b]->r.x == w.inserts[a]->r.x)
+                b++;
+            std::vector<uint64_t> out(b - a);
+            const int st = kache_amd_insert_batches(t, user, d + w.insert_at, w.insert_bounds.data() + a, b - a, w.inserts[a]->r.op,
+                                                    w.inserts[a]->r.x, out.data());
+            for(size_t i = a; i < b; ++i)
+            {
+                w.inserts[i]->status = st;
+                if(st == KACHE_OK && w.inserts[i]->r.counts)
+                    *w.inserts[i]->r.counts = out[i - a];
+            }
+            a = b;
+        }
+        finish(w.inserts);
     }
 };
 

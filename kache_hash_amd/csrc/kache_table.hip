@@ -1296,6 +1296,39 @@ struct kache_table
         }
     }
 
+    // The segments of sequences already in a device text: sequence i spans
+    // [off[i], off[i + 1] - 1) and its separator is at off[i + 1] - 1.
+    static Stage stage_of_offsets(const uint64_t* off, size_t n)
+    {
+        Stage st;
+        st.text_len = off[n];
+        st.seg_start.assign(off, off + n + 1);
+        st.seg_q.resize(n);
+        st.seg_off.assign(n, 0);
+        for(size_t i = 0; i < n; ++i)
+            st.seg_q[i] = static_cast<uint32_t>(i);
+        return st;
+    }
+
+    template <int MODE>
+    void per_sequence_device(const uint8_t* text, const uint64_t* off, size_t n, uint64_t* out, uint64_t mf, uint64_t mr) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        std::fill(out, out + n, 0);
+        if(n == 0)
+            return;
+        d_out.reserve(n * 8);
+        HIP(hipMemsetAsync(d_out.p, 0, n * 8, stream));
+        const Stage st = stage_of_offsets(off, n);
+        QueryArgs a{};
+        upload_segments(st, a);
+        a.per_seq = d_out.as<uint64_t>();
+        a.mask_fwd = mf, a.mask_rev = mr;
+        launch_query<MODE>(text, st.text_len, a);
+        HIP(hipMemcpyAsync(out, d_out.p, n * 8, hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+
     // Where a group of consecutive sequences [first, next group's first)
     // puts its lookup outputs (values and fwd may be null).
     struct LookupDest
@@ -1315,13 +1348,19 @@ struct kache_table
 
     // One lookup pass over all sequences; each group's outputs, laid out as
     // a lookup of its own sequences alone, are copied to its destinations.
-    void lookup_groups(const char* const* seqs, const size_t* lens, size_t n, const LookupDest* g, size_t ng) const
+    // With `dev_text`, the sequences are already there, at offsets `dev_off`
+    // (as per_sequence_device takes them), and seqs / lens are unused.
+    void lookup_groups(const char* const* seqs, const size_t* lens, size_t n, const LookupDest* g, size_t ng,
+                       const uint8_t* dev_text = nullptr, const uint64_t* dev_off = nullptr) const
     {
         std::lock_guard<std::mutex> lock(mu);
         std::vector<uint64_t> o(n + 1);
         uint64_t p = 0;
         for(size_t i = 0; i < n; ++i)
-            o[i] = p, p += (lens[i] >= static_cast<size_t>(K) ? lens[i] - K + 1 : 0);
+        {
+            const uint64_t len = dev_text ? dev_off[i + 1] - dev_off[i] - 1 : lens[i];
+            o[i] = p, p += (len >= static_cast<uint64_t>(K) ? len - K + 1 : 0);
+        }
         o[n] = p;
         if(p == 0)
             return;
@@ -1342,11 +1381,8 @@ struct kache_table
         if(fwd)
             fill(d_out3.p, p, 0);
 
-        SeqCursor cur;
-        Stage st;
-        while(stage_next(seqs, lens, n, cur, st))
+        const auto launch = [&](const uint8_t* text, const Stage& st)
         {
-            const uint8_t* text = upload(st);
             QueryArgs a{};
             upload_segments(st, a);
             a.out_off = d_off.as<uint64_t>();
@@ -1356,6 +1392,15 @@ struct kache_table
             a.vbytes = vb;
             launch_query<Q_LOOKUP>(text, st.text_len, a);
             sync();
+        };
+        if(dev_text)
+            launch(dev_text, stage_of_offsets(dev_off, n));
+        else
+        {
+            SeqCursor cur;
+            Stage st;
+            while(stage_next(seqs, lens, n, cur, st))
+                launch(upload(st), st);
         }
         for(size_t i = 0; i < ng; ++i)
         {
@@ -1701,6 +1746,58 @@ int kache_lookup(const kache_table* t, const char* const* seqs, const size_t* le
     if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && found == nullptr) || (t->value_bytes == 0 && values != nullptr))
         return KACHE_EINVAL;
     return guard([&](){ t->lookup(seqs, lens, n, found, values, fwd); return static_cast<int>(KACHE_OK); });
+}
+
+namespace
+{
+
+// Offsets of sequences in a device text: nondecreasing by at least 1 (each
+// sequence then its separator), within the text.
+bool offsets_ok(const char* text, uint64_t text_bytes, const uint64_t* off, size_t n)
+{
+    if(n > 0 && (off == nullptr || !device_text_ok(text, text_bytes) || text == nullptr))
+        return false;
+    for(size_t i = 0; i < n; ++i)
+        if(off[i + 1] < off[i] + 1)
+            return false;
+    return n == 0 || off[n] <= text_bytes;
+}
+
+}
+
+int kache_amd_count_seqs_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n, uint64_t* counts) noexcept
+{
+    if(t == nullptr || !offsets_ok(text, text_bytes, offsets, n) || (n > 0 && counts == nullptr))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        t->per_sequence_device<Q_COUNT>(reinterpret_cast<const uint8_t*>(text), offsets, n, counts, 0, 0);
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_lookup_groups_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
+                                   const uint64_t* group_first, size_t n_groups, uint8_t* const* found, void* const* values,
+                                   uint8_t* const* fwd) noexcept
+{
+    if(t == nullptr || !offsets_ok(text, text_bytes, offsets, n) || (n_groups > 0 && (group_first == nullptr || found == nullptr)))
+        return KACHE_EINVAL;
+    for(size_t i = 0; i < n_groups; ++i)
+        if(group_first[i] > n || (i > 0 && group_first[i] < group_first[i - 1]) || found[i] == nullptr ||
+           (values && values[i] && t->value_bytes == 0))
+            return KACHE_EINVAL;
+    if(n_groups > 0 && group_first[0] != 0)
+        return KACHE_EINVAL;
+    if(n_groups == 0)
+        return n == 0 ? static_cast<int>(KACHE_OK) : static_cast<int>(KACHE_EINVAL);
+    return guard([&]()
+    {
+        std::vector<kache_table::LookupDest> d(n_groups);
+        for(size_t i = 0; i < n_groups; ++i)
+            d[i] = {group_first[i], found[i], values ? values[i] : nullptr, fwd ? fwd[i] : nullptr};
+        t->lookup_groups(nullptr, nullptr, n, d.data(), n_groups, reinterpret_cast<const uint8_t*>(text), offsets);
+        return static_cast<int>(KACHE_OK);
+    });
 }
 
 int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, const uint64_t* group_first,

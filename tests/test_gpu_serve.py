@@ -256,3 +256,38 @@ def test_lookup_groups_equals_separate_lookups(api):
     bad = np.array([1, 0], np.uint64)
     one = (C.c_void_p * 2)(1, 1)
     assert api.lib.kache_amd_lookup_groups(t.h, b.c_seqs, b.c_lens, b.n, bad.ctypes.data, 2, one, None, None) == capi.KACHE_EINVAL
+
+
+def test_device_text_queries_equal_host_queries(api):
+    """kache_amd_count_seqs_device / kache_amd_lookup_groups_device on a device
+    text give kache_count / kache_lookup of the same sequences."""
+    import ctypes as C
+    g = G.genome(80_000, 58)
+    rng = np.random.default_rng(58)
+    t = capi.Table(api, 4, expected_keys=100_000)
+    u = t.register()
+    t.insert(random_seqs(rng, g, 200), capi.ADD, 3, u)
+    seqs = random_seqs(rng, g, 60)
+    text = b"".join(x + b"\n" for x in seqs)
+    off = np.concatenate([[0], np.cumsum([len(x) + 1 for x in seqs])]).astype(np.uint64)
+    d = capi.DeviceBuffer(api, len(text) + capi.TEXT_PAD)
+    d.upload(text + b"\n" * capi.TEXT_PAD)
+    out = np.zeros(len(seqs), np.uint64)
+    api.check(api.lib.kache_amd_count_seqs_device(t.h, d.ptr, len(text), off.ctypes.data, len(seqs), out.ctypes.data), "count")
+    assert np.array_equal(out, t.count(seqs))
+    groups = [seqs[:10], seqs[10:10], seqs[10:45], seqs[45:]]
+    first = np.array([0, 10, 10, 45], np.uint64)
+    outs = [capi.make_request(4, capi.REQ_LOOKUP, grp)[1] for grp in groups]
+    P = C.c_void_p * 4
+    api.check(api.lib.kache_amd_lookup_groups_device(t.h, d.ptr, len(text), off.ctypes.data, len(seqs), first.ctypes.data, 4,
+                                                     P(*[o["found"].ctypes.data for o in outs]),
+                                                     P(*[o["values"].ctypes.data for o in outs]),
+                                                     P(*[o["fwd"].ctypes.data for o in outs])), "lookup")
+    for grp, o in zip(groups, outs):
+        f, v, fw = t.lookup(grp)
+        p = len(f)
+        assert np.array_equal(o["found"][:p], f) and np.array_equal(o["fwd"][:p], fw)
+        assert np.array_equal(o["values"][:4 * p].view(np.uint32).astype(np.uint64), v)
+    bad = off.copy()
+    bad[3] = bad[2]                       # an empty span: no room for the separator
+    assert api.lib.kache_amd_count_seqs_device(t.h, d.ptr, len(text), bad.ctypes.data, len(seqs), out.ctypes.data) == capi.KACHE_EINVAL
