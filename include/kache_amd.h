@@ -101,6 +101,17 @@ int kache_amd_read_fastx(const char* path, kache_amd_text_fn fn, void* ctx) KACH
 int kache_amd_insert_fastx(kache_table* t, uint32_t user, const char* path, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
 int kache_amd_update_fastx(kache_table* t, const char* path, uint32_t op, uint64_t x, uint64_t* present) KACHE_NOEXCEPT;
 
+/* Persistence of a table: kache_amd_save writes its device image (bucket
+ * store and overflow table, as they are) with its host state (size, growth
+ * schedule, registered users) to `path`; kache_amd_load makes a new table
+ * from such a file, equal to the saved one in every result and in its
+ * future growth.  The file is this library's own format (a versioned
+ * header with hashes of itself and of the image), not the reference's,
+ * which has none.  EINVAL for a file that cannot be opened, is not such a
+ * file, or fails its hashes; ENOMEM when the table does not fit. */
+int kache_amd_save(const kache_table* t, const char* path) KACHE_NOEXCEPT;
+int kache_amd_load(const char* path, kache_table** out) KACHE_NOEXCEPT;
+
 /* Histogram of a map's values, as the reference's counter computes it
  * (counter/Kmer_Set_Counter.cpp:97-120): out[c] = the number of keys with
  * value c for c in [1, max], out[0] = 0; `out` holds max + 1 entries.  One

@@ -111,6 +111,8 @@ AMD_SYMBOLS = {
     "kache_amd_insert_text": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_update_text": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_histogram": (C.c_int, [_P, C.c_uint64, _P]),
+    "kache_amd_save": (C.c_int, [_P, C.c_char_p]),
+    "kache_amd_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "kache_amd_insert_seqs": (C.c_int, [_P, C.c_uint32, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_update_seqs": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_read_fastx": (C.c_int, [C.c_char_p, TEXT_FN, _P]),
@@ -408,6 +410,21 @@ class Table:
         n = C.c_uint64()
         self.api.check(self.api.lib.kache_amd_update_fastx(self.h, os.fsencode(path), op, x, C.byref(n)), "kache_amd_update_fastx")
         return n.value
+
+    def save(self, path):
+        self.api.check(self.api.lib.kache_amd_save(self.h, os.fsencode(path)), "kache_amd_save")
+
+    @classmethod
+    def load(cls, api, path):
+        """A table made from a file written by save()."""
+        h = C.c_void_p()
+        api.check(api.lib.kache_amd_load(os.fsencode(path), C.byref(h)), "kache_amd_load")
+        t = cls.__new__(cls)
+        t.api, t.h = api, h
+        i = Info()
+        api.check(api.lib.kache_amd_get_info(h, C.byref(i)), "kache_amd_get_info")
+        t.width = i.value_bytes
+        return t
 
     def histogram(self, max_count):
         """out[c] = keys with value c, c in [1, max_count]; out[0] = 0."""

@@ -12,6 +12,10 @@
 #include "kache_kernels.hpp"
 
 #include <algorithm>
+#include <cerrno>
+#include <cstddef>
+#include <cstdio>
+#include <memory>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -1551,6 +1555,140 @@ struct kache_table
         out[0] = 0;
     }
 
+    // --- Persistence: the device image with its host state ---
+
+    struct FileHeader
+    {
+        char magic[8];
+        uint32_t version;
+        uint32_t value_bytes;
+        double lf;
+        uint64_t cap, ov_slots, ov_count, size, resizes, approx_sz, resize_th, step;
+        uint32_t user_c, reserved;
+        uint64_t checkpoint[KACHE_MAX_USERS];
+        uint64_t data_bytes;
+        uint64_t data_hash;
+        uint64_t header_hash;       // of the bytes before it
+    };
+    static constexpr char MAGIC[8] = {'K', 'A', 'M', 'D', 'T', 'B', 'L', '1'};
+    static constexpr uint64_t IO_CHUNK = uint64_t(64) << 20;
+
+    // A word-wise hash of file bytes (a multiply-rotate mix per 8 bytes).
+    static uint64_t hash_words(uint64_t h, const void* p, uint64_t n)
+    {
+        const uint64_t* w = static_cast<const uint64_t*>(p);
+        for(uint64_t i = 0; i < n / 8; ++i)
+        {
+            h ^= w[i];
+            h *= 0x9E3779B97F4A7C15ull;
+            h = (h << 29) | (h >> 35);
+        }
+        return h;
+    }
+
+    // The device regions of the image, in file order.
+    std::vector<std::pair<uint8_t*, uint64_t>> regions() const
+    {
+        return {{buckets, cap * bucket_bytes()}, {reinterpret_cast<uint8_t*>(ov), ov_slots * slot_words() * 8}};
+    }
+
+    void save(const char* path) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        sync();
+        FileHeader h{};
+        std::memcpy(h.magic, MAGIC, 8);
+        h.version = 1, h.value_bytes = value_bytes, h.lf = lf;
+        h.cap = cap, h.ov_slots = ov_slots, h.ov_count = ov_count, h.size = size, h.resizes = resizes;
+        h.approx_sz = approx_sz, h.resize_th = resize_th, h.step = step;
+        h.user_c = user_c.load();
+        std::memcpy(h.checkpoint, checkpoint, sizeof(checkpoint));
+        for(const auto& [p, n] : regions())
+            h.data_bytes += n;
+        FILE* f = std::fopen(path, "wb");
+        if(!f)
+            throw std::invalid_argument(std::string("cannot create ") + path + ": " + std::strerror(errno));
+        std::unique_ptr<FILE, int (*)(FILE*)> guard_f(f, std::fclose);
+        if(std::fwrite(&h, sizeof(h), 1, f) != 1)
+            throw std::runtime_error("write failed");
+        HostBuf buf;
+        buf.reserve(IO_CHUNK);
+        uint64_t dh = 0x6B616368655F616Dull;
+        for(const auto& [p, n] : regions())
+            for(uint64_t o = 0; o < n; o += IO_CHUNK)
+            {
+                const uint64_t c = std::min(IO_CHUNK, n - o);
+                HIP(hipMemcpyAsync(buf.p, p + o, c, hipMemcpyDeviceToHost, stream));
+                HIP(hipStreamSynchronize(stream));
+                dh = hash_words(dh, buf.p, c);
+                if(std::fwrite(buf.p, 1, c, f) != c)
+                    throw std::runtime_error("write failed");
+            }
+        h.data_hash = dh;
+        h.header_hash = hash_words(0x6B616368655F6864ull, &h, offsetof(FileHeader, header_hash));
+        if(std::fseek(f, 0, SEEK_SET) != 0 || std::fwrite(&h, sizeof(h), 1, f) != 1)
+            throw std::runtime_error("write failed");
+        if(std::fflush(f) != 0)
+            throw std::runtime_error("write failed");
+    }
+
+    // Reads and checks a file's header (no device work).
+    static FileHeader read_header(FILE* f)
+    {
+        FileHeader h{};
+        if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, MAGIC, 8) != 0 || h.version != 1)
+            throw std::invalid_argument("not a table file of this version");
+        if(h.header_hash != hash_words(0x6B616368655F6864ull, &h, offsetof(FileHeader, header_hash)))
+            throw std::invalid_argument("the table file's header is corrupt");
+        const uint32_t vb = h.value_bytes;
+        const uint64_t bb = vb ? Geo<true>::bucket_bytes : Geo<false>::bucket_bytes;
+        if(!(vb == 0 || vb == 1 || vb == 2 || vb == 4 || vb == 8) || !(h.lf > 0 && h.lf < 1) || h.cap == 0 || h.cap > MAX_CAP ||
+           h.ov_slots == 0 || (h.ov_slots & (h.ov_slots - 1)) || h.user_c > KACHE_MAX_USERS ||
+           h.data_bytes != h.cap * bb + h.ov_slots * (vb ? 2 : 1) * 8)
+            throw std::invalid_argument("the table file's header is inconsistent");
+        return h;
+    }
+
+    // Replaces this (new, empty) table's store by the file's image and state.
+    void load(FILE* f, const FileHeader& h)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        sync();
+        uint8_t* nb = dev_alloc<uint8_t>(h.cap * bucket_bytes());
+        uint64_t* nov = nullptr;
+        try
+        {
+            nov = dev_alloc<uint64_t>(h.ov_slots * slot_words());
+        }
+        catch(...)
+        {
+            (void)hipFree(nb);
+            throw;
+        }
+        HIP(hipFree(buckets));
+        HIP(hipFree(ov));
+        buckets = nb, ov = nov, cap = h.cap, ov_slots = h.ov_slots;
+        HostBuf buf;
+        buf.reserve(IO_CHUNK);
+        uint64_t dh = 0x6B616368655F616Dull;
+        for(const auto& [p, n] : regions())
+            for(uint64_t o = 0; o < n; o += IO_CHUNK)
+            {
+                const uint64_t c = std::min(IO_CHUNK, n - o);
+                if(std::fread(buf.p, 1, c, f) != c)
+                    throw std::invalid_argument("the table file is truncated");
+                dh = hash_words(dh, buf.p, c);
+                HIP(hipMemcpyAsync(p + o, buf.p, c, hipMemcpyHostToDevice, stream));
+                HIP(hipStreamSynchronize(stream));
+            }
+        if(dh != h.data_hash)
+            throw std::invalid_argument("the table file's data is corrupt");
+        ov_count = h.ov_count, size = h.size, resizes = h.resizes;
+        approx_sz = h.approx_sz, resize_th = h.resize_th, step = h.step;
+        std::memcpy(checkpoint, h.checkpoint, sizeof(checkpoint));
+        user_c.store(h.user_c);
+    }
+
     // capi/Kmer_Table.hpp:270-279 with the placement of
     // include/Streaming_Kmer_Hash_Table.hpp:2728-2752.
     void stats(kache_stats& s) const
@@ -1875,6 +2013,73 @@ int kache_amd_update_text(kache_table* t, const char* text, uint64_t text_bytes,
             *present = c;
         return static_cast<int>(KACHE_OK);
     });
+}
+
+int kache_amd_save(const kache_table* t, const char* path) noexcept
+{
+    if(t == nullptr || path == nullptr)
+        return KACHE_EINVAL;
+    try
+    {
+        t->save(path);
+        return KACHE_OK;
+    }
+    catch(const std::invalid_argument& e)
+    {
+        last_error = e.what();
+        return KACHE_EINVAL;
+    }
+    catch(...)
+    {
+        return guard([]() -> int { throw; });
+    }
+}
+
+int kache_amd_load(const char* path, kache_table** out) noexcept
+{
+    if(path == nullptr || out == nullptr)
+        return KACHE_EINVAL;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if(!f)
+    {
+        last_error = std::string("cannot open ") + path + ": " + std::strerror(errno);
+        return KACHE_EINVAL;
+    }
+    std::unique_ptr<FILE, int (*)(FILE*)> guard_f(f, std::fclose);
+    kache_table::FileHeader h{};
+    try
+    {
+        h = kache_table::read_header(f);
+    }
+    catch(const std::exception& e)
+    {
+        last_error = e.what();
+        return KACHE_EINVAL;
+    }
+    kache_options o = kache_default_options(31, h.value_bytes);
+    o.expected_keys = 1, o.load_factor = h.lf;
+    kache_table* t = nullptr;
+    const int st = kache_new(&o, &t);
+    if(st != KACHE_OK)
+        return st;
+    try
+    {
+        t->load(f, h);
+    }
+    catch(const std::invalid_argument& e)
+    {
+        kache_free(t);
+        last_error = e.what();
+        return KACHE_EINVAL;
+    }
+    catch(...)
+    {
+        kache_free(t);
+        return guard([]() -> int { throw; });
+    }
+    *out = t;
+    return KACHE_OK;
 }
 
 int kache_amd_histogram(const kache_table* t, uint64_t max, uint64_t* out) noexcept

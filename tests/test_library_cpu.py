@@ -172,3 +172,14 @@ def test_host_fits_matches_oracle(api):
         for lf in (0.5, 0.8, 0.9):
             for pow2 in (0, 1):
                 assert bool(api.lib.kache_amd_host_fits(n, lf, pow2)) == O.fits(n, lf, bool(pow2)), (n, lf, pow2)
+
+
+def test_load_rejects_bad_files_before_touching_a_device(api, tmp_path):
+    h = C.c_void_p(1234)
+    assert api.lib.kache_amd_load(str(tmp_path / "absent").encode(), C.byref(h)) == capi.KACHE_EINVAL and not h.value
+    junk = tmp_path / "junk"
+    junk.write_bytes(b"KAMDTBL1" + b"\0" * 2000)
+    assert api.lib.kache_amd_load(str(junk).encode(), C.byref(h)) == capi.KACHE_EINVAL
+    assert "header" in api.last_error() or "version" in api.last_error()
+    assert api.lib.kache_amd_load(None, C.byref(h)) == capi.KACHE_EINVAL
+    assert api.lib.kache_amd_save(None, b"x") == capi.KACHE_EINVAL
