@@ -13,6 +13,9 @@
 
 #include "kache_amd.h"
 
+// The library's packing threads (kache_table.hip).
+void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n);
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -418,17 +421,25 @@ struct kache_amd_server
             slot_bytes[sl] = want;
         }
         char* h = h_slot[sl];
-        std::memset(h, '\n', need);
+        // The sequences' copies (each followed by '\n') are collected, then
+        // made by the library's packing threads; the gaps that align the
+        // regions and the final pad are separators too.
+        std::vector<char*> dst;
+        std::vector<const char*> src;
+        std::vector<uint64_t> len;
+        uint64_t filled = 0;
         const auto put = [&](const kache_amd_request& r, uint64_t at, std::vector<uint64_t>* offs)
         {
+            if(at > filled)
+                std::memset(h + filled, '\n', at - filled);
             for(size_t j = 0; j < r.n; ++j)
             {
                 if(offs)
                     offs->push_back(at);
-                if(r.lens[j])
-                    std::memcpy(h + at, r.seqs[j], r.lens[j]);
+                dst.push_back(h + at), src.push_back(r.seqs[j]), len.push_back(r.lens[j]);
                 at += r.lens[j] + 1;
             }
+            filled = at;
             return at;
         };
         uint64_t at = w.count_at;
@@ -452,6 +463,9 @@ struct kache_amd_server
             at = put(q->r, at, nullptr);
             w.insert_bounds.push_back(at - w.insert_at);
         }
+        if(need > filled)
+            std::memset(h + filled, '\n', need - filled);
+        kamd_copy_spans(dst.data(), src.data(), len.data(), dst.size());
         return kache_amd_copy(d_slot[sl], h, need, 1);
     }
 

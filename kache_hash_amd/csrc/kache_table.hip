@@ -28,6 +28,9 @@
 #include <string>
 #include <chrono>
 #include <vector>
+#include <condition_variable>
+#include <functional>
+#include <thread>
 
 using namespace kamd;
 
@@ -178,6 +181,110 @@ struct HostBuf
     ~HostBuf() { if(p) (void)hipHostFree(p); }
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
+
+// Host threads that pack sequences into the pinned staging buffers: one
+// packing thread copies at a fraction of the link's speed, so large stages
+// are copied by several.  One job at a time (callers queue on `run_mu`);
+// the calling thread works too.
+class CopyPool
+{
+    std::vector<std::thread> workers;
+    std::mutex mu, run_mu;
+    std::condition_variable cv_job, cv_done;
+    const std::function<void(size_t)>* job = nullptr;
+    size_t n_tasks = 0, next = 0, finished = 0;
+    uint64_t gen = 0;
+    bool stop = false;
+
+    // Runs tasks of the current job until none is left; returns with `lk` held.
+    void drain(std::unique_lock<std::mutex>& lk)
+    {
+        while(next < n_tasks)
+        {
+            const size_t i = next++;
+            const auto* f = job;
+            lk.unlock();
+            (*f)(i);
+            lk.lock();
+            if(++finished == n_tasks)
+                cv_done.notify_all();
+        }
+    }
+
+    explicit CopyPool(unsigned n)
+    {
+        for(unsigned i = 0; i < n; ++i)
+            workers.emplace_back([this]
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                uint64_t seen = 0;
+                while(true)
+                {
+                    cv_job.wait(lk, [&]{ return stop || gen != seen; });
+                    if(stop)
+                        return;
+                    seen = gen;
+                    drain(lk);
+                }
+            });
+    }
+
+public:
+    static constexpr unsigned THREADS = 8;
+
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_job.notify_all();
+        for(auto& w : workers)
+            w.join();
+    }
+
+    static CopyPool& get()
+    {
+        static CopyPool pool(THREADS - 1);
+        return pool;
+    }
+
+    void run(size_t n, const std::function<void(size_t)>& f)
+    {
+        if(n <= 1)
+        {
+            for(size_t i = 0; i < n; ++i)
+                f(i);
+            return;
+        }
+        std::lock_guard<std::mutex> one(run_mu);
+        std::unique_lock<std::mutex> lk(mu);
+        job = &f, n_tasks = n, next = 0, finished = 0, gen++;
+        cv_job.notify_all();
+        drain(lk);
+        cv_done.wait(lk, [&]{ return finished == n_tasks; });
+        job = nullptr;
+    }
+};
+
+// Copies n bytes, in parallel when large.
+void par_copy(char* dst, const char* src, uint64_t n)
+{
+    constexpr uint64_t PIECE = uint64_t(2) << 20;
+    if(n < 2 * PIECE)
+    {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const uint64_t parts = std::min<uint64_t>(CopyPool::THREADS, n / PIECE);
+    const uint64_t each = (n + parts - 1) / parts;
+    CopyPool::get().run(parts, [&](size_t i)
+    {
+        const uint64_t a = i * each, b = std::min(n, a + each);
+        if(a < b)
+            std::memcpy(dst + a, src + a, b - a);
+    });
+}
 
 // Sequences staged for one device pass: segments of the caller's sequences,
 // each followed by a separator byte (include/Streaming_Kmer_Hash_Table.hpp:1697-1709
@@ -613,9 +720,7 @@ struct kache_table
             if(rest + 1 <= room)
             {
                 st.seg_start.push_back(st.text_len), st.seg_q.push_back(static_cast<uint32_t>(q)), st.seg_off.push_back(cur.off);
-                std::memcpy(out + st.text_len, seqs[q] + cur.off, rest);
-                st.text_len += rest;
-                out[st.text_len++] = '\n';
+                st.text_len += rest + 1;
                 cur.q++, cur.off = 0;
                 continue;
             }
@@ -624,15 +729,49 @@ struct kache_table
                 // A piece of the sequence; the next piece starts k - 1 characters before its end.
                 const uint64_t take = room - 1;
                 st.seg_start.push_back(st.text_len), st.seg_q.push_back(static_cast<uint32_t>(q)), st.seg_off.push_back(cur.off);
-                std::memcpy(out + st.text_len, seqs[q] + cur.off, take);
-                st.text_len += take;
-                out[st.text_len++] = '\n';
+                st.text_len += take + 1;
                 cur.off += take - (K - 1);
             }
             break;
         }
-        std::memset(out + st.text_len, '\n', TEXT_PAD);
         st.seg_start.push_back(st.text_len);
+        // The copies: segment i fills [seg_start[i], seg_start[i + 1] - 1), then '\n'.
+        const size_t ns = st.seg_q.size();
+        const auto copy = [&](size_t a, size_t b)
+        {
+            for(size_t i = a; i < b; ++i)
+            {
+                const uint64_t len = st.seg_start[i + 1] - st.seg_start[i] - 1;
+                std::memcpy(out + st.seg_start[i], seqs[st.seg_q[i]] + st.seg_off[i], len);
+                out[st.seg_start[i] + len] = '\n';
+            }
+        };
+        if(st.text_len < (uint64_t(4) << 20) || ns < 2)
+        {
+            if(ns == 1 && st.text_len >= (uint64_t(4) << 20))
+            {
+                par_copy(out, seqs[st.seg_q[0]] + st.seg_off[0], st.text_len - 1);
+                out[st.text_len - 1] = '\n';
+            }
+            else
+                copy(0, ns);
+        }
+        else
+        {
+            // Ranges of segments of about equal bytes, one per task.
+            const size_t parts = CopyPool::THREADS;
+            std::vector<size_t> cut(parts + 1, ns);
+            cut[0] = 0;
+            for(size_t p = 1, i = 0; p < parts; ++p)
+            {
+                const uint64_t target = st.text_len * p / parts;
+                while(i < ns && st.seg_start[i] < target)
+                    ++i;
+                cut[p] = i;
+            }
+            CopyPool::get().run(parts, [&](size_t p){ copy(cut[p], std::max(cut[p], cut[p + 1])); });
+        }
+        std::memset(out + st.text_len, '\n', TEXT_PAD);
         return true;
     }
 
@@ -927,7 +1066,7 @@ struct kache_table
             }
             hb.reserve(end - off + 1 + TEXT_PAD);
             char* out = hb.as<char>();
-            std::memcpy(out, text + off, end - off);
+            par_copy(out, text + off, end - off);
             out[end - off] = '\n';
             std::memset(out + end - off + 1, '\n', TEXT_PAD);
             st.text_len = end - off + 1;
@@ -2423,6 +2562,47 @@ int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key
     return KACHE_OK;
 }
 
+}
+
+// Copies n spans (each followed by a '\n' at dst[i] + len[i]), split over
+// the packing threads by bytes (other sources of the library).
+__attribute__((visibility("hidden"))) void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n)
+{
+    uint64_t total = 0;
+    for(size_t i = 0; i < n; ++i)
+        total += len[i] + 1;
+    const auto copy = [&](size_t a, size_t b)
+    {
+        for(size_t i = a; i < b; ++i)
+        {
+            if(len[i])
+                std::memcpy(dst[i], src[i], len[i]);
+            dst[i][len[i]] = '\n';
+        }
+    };
+    if(total < (uint64_t(4) << 20) || n < 2)
+    {
+        if(n == 1 && total >= (uint64_t(4) << 20))
+        {
+            par_copy(dst[0], src[0], len[0]);
+            dst[0][len[0]] = '\n';
+        }
+        else
+            copy(0, n);
+        return;
+    }
+    const size_t parts = CopyPool::THREADS;
+    std::vector<size_t> cut(parts + 1, n);
+    cut[0] = 0;
+    uint64_t acc = 0;
+    for(size_t p = 1, i = 0; p < parts; ++p)
+    {
+        const uint64_t target = total * p / parts;
+        while(i < n && acc < target)
+            acc += len[i] + 1, ++i;
+        cut[p] = i;
+    }
+    CopyPool::get().run(parts, [&](size_t p){ copy(cut[p], std::max(cut[p], cut[p + 1])); });
 }
 
 // Sets the calling thread's last error (other sources of the library).
