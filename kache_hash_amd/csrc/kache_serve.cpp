@@ -196,8 +196,31 @@ struct kache_amd_server
             const double t0 = now_s();
             for(auto& q : wave)
                 q->t_start = t0, q->wave = wave_id, q->phase = phase_of(q->r.kind);
-            run(wave);
+            try
+            {
+                run(wave);
+            }
+            catch(...)
+            {
+                fail(wave);
+            }
         }
+    }
+
+    // Completes the requests of a wave that could not run (host memory ran
+    // out): each gets KACHE_ENOMEM unless it already completed.
+    void fail(const std::vector<std::shared_ptr<Req>>& wave)
+    {
+        std::vector<Req*> open;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            for(auto& q : wave)
+                if(!q->done)
+                    open.push_back(q.get());
+        }
+        for(Req* q : open)
+            q->status = KACHE_ENOMEM;
+        finish(open);
     }
 
     // Marks requests done as soon as their phase ends: later phases of the
@@ -490,7 +513,15 @@ struct kache_amd_server
                 slot_busy[w->slot] = true;
                 w->reqs = take_wave(w->id);
             }
-            const int st = pack(*w);
+            int st = KACHE_ENOMEM;
+            try
+            {
+                st = pack(*w);
+            }
+            catch(...)
+            {
+                st = KACHE_ENOMEM;
+            }
             if(st != KACHE_OK)
             {
                 for(auto& q : w->reqs)
@@ -523,7 +554,14 @@ struct kache_amd_server
                 w = std::move(staged.front());
                 staged.pop_front();
             }
-            execute(*w);
+            try
+            {
+                execute(*w);
+            }
+            catch(...)
+            {
+                fail(w->reqs);
+            }
             std::lock_guard<std::mutex> lock(mu);
             slot_busy[w->slot] = false;
             cv_slot.notify_all();
