@@ -32,6 +32,26 @@ PEAK_HBM_GBS = 8000.0
 ALG_BYTES_PER_KMER = 150 / 120 + 64 / 5.29 + 16
 
 
+# Bucket lines touched per k-mer of a u64 count map (the 64-byte metadata line
+# per 5.29-k-mer minimizer run, a quarter of a 64-byte line of 16-byte slots),
+# as the random-access roofline counts them.
+LINES_PER_KMER = 1 / 5.29 + 16 / 64
+
+
+def random_line_roof(kmers_per_s):
+    """The insert kernel against the measured random read-modify-write roof of
+    64-byte lines (profiles/memory_roof_mi355x.json, made by
+    kache_hash_amd/memory_roof on an MI355X)."""
+    path = os.path.join(ROOT, "profiles", "memory_roof_mi355x.json")
+    try:
+        roof = json.load(open(path))["random_update_64_lines_per_s"]
+    except (OSError, KeyError, ValueError):
+        return None
+    lines = kmers_per_s * LINES_PER_KMER
+    return {"lines_per_kmer": LINES_PER_KMER, "achieved_lines_per_s": lines, "roof": "random_update_64",
+            "roof_lines_per_s": roof, "frac": lines / roof}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -247,7 +267,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "k_insert<map, ADD>", "alg_bytes_per_kmer": ALG_BYTES_PER_KMER,
-                     "kmers_per_launch": per_launch_kmers, "avg_launch_ms": avg_ms},
+                     "kmers_per_launch": per_launch_kmers, "avg_launch_ms": avg_ms,
+                     "random_line_roof": random_line_roof(per_launch_kmers / (avg_ms * 1e-3) if avg_ms > 0 else 0.0)},
         "cpu_baseline": cpu,
     }
     if cpu:
