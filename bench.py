@@ -148,8 +148,8 @@ def cpu_baseline(host_text, n_reads, expected, threads):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
-              "the sharding design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): there is no multi-GPU path; the table "
+              "would shard by contiguous bucket ranges, one all-to-all of super-k-mers per batch", file=sys.stderr)
         sys.exit(2)
 
     import kache_hash_amd as KA
