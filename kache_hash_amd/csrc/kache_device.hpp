@@ -229,8 +229,19 @@ template <bool MAP> struct Geo
 template <bool MAP>
 __device__ __forceinline__ uint8_t* bucket_at(const TableView& T, uint64_t b) { return T.buckets + b * Geo<MAP>::bucket_bytes; }
 
+// Slots are split into words: a bucket holds its 32 stored keys, then (in a
+// map) their 32 values, so the value of the key at word k is word k + VAL_OFF.
+// The overflow table keeps the same shape in blocks of 32 slots.
+constexpr uint32_t VAL_OFF = B;
+
 template <bool MAP>
-__device__ __forceinline__ uint64_t* slot_at(uint8_t* bucket, uint32_t j) { return reinterpret_cast<uint64_t*>(bucket + META_BYTES) + j * Geo<MAP>::slot_words; }
+__device__ __forceinline__ uint64_t* slot_at(uint8_t* bucket, uint32_t j) { return reinterpret_cast<uint64_t*>(bucket + META_BYTES) + j; }
+
+template <bool MAP>
+__host__ __device__ __forceinline__ uint64_t* ov_slot(uint64_t* ov, uint64_t i)
+{
+    return MAP ? ov + (i & ~uint64_t(B - 1)) * 2 + (i & (B - 1)) : ov + i;
+}
 
 enum Res : int { RES_FOUND = 0, RES_NEW = 1, RES_FULL = 2 };
 
@@ -295,7 +306,7 @@ __device__ __forceinline__ Res ov_insert(const TableView& T, uint64_t key, uint6
     uint64_t i = xxh3_8(key, 0) & T.ov_mask;
     for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
     {
-        uint64_t* s = T.ov + i * Geo<MAP>::slot_words;
+        uint64_t* s = ov_slot<MAP>(T.ov, i);
         uint64_t cur = ld_acq(s);
         if(cur == sk)
             return slot = s, RES_FOUND;
@@ -316,7 +327,7 @@ __device__ __forceinline__ uint64_t* ov_find(const TableView& T, uint64_t key, u
     uint64_t i = xxh3_8(key, 0) & T.ov_mask;
     for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
     {
-        uint64_t* s = T.ov + i * Geo<MAP>::slot_words;
+        uint64_t* s = ov_slot<MAP>(T.ov, i);
         const uint64_t cur = *s;
         if(cur == sk)
             return s;

@@ -50,7 +50,7 @@ __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restri
         {
             ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
             if constexpr(MAP)
-                value_op<OP>(slot + 1, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
+                value_op<OP>(slot + VAL_OFF, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
         }
     });
     block_add(ins, &cnt->inserted, red);
@@ -71,7 +71,7 @@ __global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restri
             if(uint64_t* slot = find_key<true>(T, d.key, d.h, d.c))
             {
                 present++;
-                value_op<OP>(slot + 1, x, T.vmax, false);
+                value_op<OP>(slot + VAL_OFF, x, T.vmax, false);
             }
     });
     block_add(present, &cnt->present, red);
@@ -127,7 +127,7 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
         bool hit = false;
         if(d.valid)
             if(const uint64_t* slot = find_key<MAP>(T, d.key, d.h, d.c))
-                hit = true, v = MAP ? slot[1] : 0;
+                hit = true, v = MAP ? slot[VAL_OFF] : 0;
 
         if constexpr(MODE == Q_TOTAL)
             total += hit, positions += d.valid;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
             {
                 sk = os[0];
                 if(MAP)
-                    val = os[1];
+                    val = os[VAL_OFF];
                 const uint64_t h = minimizer_hash(~sk);
                 if(bucket_of(h, O.cap) == b)
                 {
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
                 uint64_t* ns = slot_at<MAP>(nb, r);
                 ns[0] = sk;
                 if(MAP)
-                    ns[1] = val;
+                    ns[VAL_OFF] = val;
                 nb[r] = c;
             }
             if(j >= n_lo)
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
                 uint64_t* ns = slot_at<MAP>(nlo, j);
                 ns[0] = 0;
                 if(MAP)
-                    ns[1] = 0;
+                    ns[VAL_OFF] = 0;
                 nlo[j] = 0;
             }
             if(j >= n_hi)
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
                 uint64_t* ns = slot_at<MAP>(nhi, j);
                 ns[0] = 0;
                 if(MAP)
-                    ns[1] = 0;
+                    ns[VAL_OFF] = 0;
                 nhi[j] = 0;
             }
             if(j < 8)   // the spill filters start clear
@@ -286,7 +286,7 @@ __device__ __forceinline__ void reinsert_one(const TableView& N, uint64_t sk, ui
     {
         ov_n += in_ov;
         if(MAP)
-            st_rel(slot + 1, val);
+            st_rel(slot + VAL_OFF, val);
     }
 }
 
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                     const uint64_t* os = slot_at<MAP>(bucket_at<MAP>(O, 2 * (ch * 64 + src) + (bit >> 5)), bit & 31);
                     sk = os[0];
                     if(MAP)
-                        val = os[1];
+                        val = os[VAL_OFF];
                 }
                 uint64_t* slot;
                 bool in_ov;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                     {
                         ov_n += in_ov;
                         if(MAP)
-                            st_rel(slot + 1, val);
+                            st_rel(slot + VAL_OFF, val);
                     }
                 }
             }
@@ -365,10 +365,10 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
             uint64_t sk = 0, val = 0;
             if(i <= O.ov_mask)
             {
-                const uint64_t* os = O.ov + i * Geo<MAP>::slot_words;
+                const uint64_t* os = ov_slot<MAP>(O.ov, i);
                 sk = os[0];
                 if(MAP && sk)
-                    val = os[1];
+                    val = os[VAL_OFF];
             }
             const bool active = sk != 0;
             uint64_t* slot;
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                 {
                     ov_n += in_ov;
                     if(MAP)
-                        st_rel(slot + 1, val);
+                        st_rel(slot + VAL_OFF, val);
                 }
             }
         }
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < old_slots; g += stride)
     {
-        const uint64_t* os = old_ov + g * Geo<MAP>::slot_words;
+        const uint64_t* os = ov_slot<MAP>(const_cast<uint64_t*>(old_ov), g);
         const uint64_t sk = os[0];
         if(!sk)
             continue;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint
         if(ov_insert<MAP>(N, ~sk, sk, slot) == RES_FULL)
             failed++;
         else if(MAP)
-            st_rel(slot + 1, os[1]);
+            st_rel(slot + VAL_OFF, os[VAL_OFF]);
     }
     block_add(failed, &cnt->failed, red);
 }
@@ -440,15 +440,15 @@ __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint
                 if(!is_key(sk))
                     sk = 0;
                 else if(MAP)
-                    val = s[1];
+                    val = s[VAL_OFF];
             }
         }
         else if(g < all)
         {
-            const uint64_t* s = T.ov + (o0 + g - main_n) * Geo<MAP>::slot_words;
+            const uint64_t* s = ov_slot<MAP>(T.ov, o0 + g - main_n);
             sk = s[0];
             if(MAP && sk)
-                val = s[1];
+                val = s[VAL_OFF];
         }
         const uint64_t m = __ballot(sk != 0);
         if(!m)
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
         }
         else
         {
-            s = T.ov + (g - main_n) * Geo<MAP>::slot_words;
+            s = ov_slot<MAP>(T.ov, g - main_n);
             if(!s[0])
                 continue;
         }
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
             acc++;
             continue;
         }
-        const uint64_t v = s[1] & mask;
+        const uint64_t v = s[VAL_OFF] & mask;
         acc += what == 0 ? (v != 0) : what == 1 ? static_cast<uint64_t>(__popcll(v)) : v;
     }
     block_add(acc, out, red);
@@ -536,11 +536,11 @@ __global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, u
         }
         else
         {
-            s = T.ov + (g - main_n) * Geo<true>::slot_words;
+            s = ov_slot<true>(T.ov, g - main_n);
             if(!s[0])
                 continue;
         }
-        const uint64_t v = s[1];
+        const uint64_t v = s[VAL_OFF];
         if(v < 1 || v > max)
             continue;
         if constexpr(LDS)

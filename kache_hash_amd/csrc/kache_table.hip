@@ -1737,7 +1737,7 @@ struct kache_table
         sync();
         FileHeader h{};
         std::memcpy(h.magic, MAGIC, 8);
-        h.version = 1, h.value_bytes = value_bytes, h.lf = lf;
+        h.version = 2, h.value_bytes = value_bytes, h.lf = lf;
         h.cap = cap, h.ov_slots = ov_slots, h.ov_count = ov_count, h.size = size, h.resizes = resizes;
         h.approx_sz = approx_sz, h.resize_th = resize_th, h.step = step;
         h.user_c = user_c.load();
@@ -1775,7 +1775,7 @@ struct kache_table
     static FileHeader read_header(FILE* f)
     {
         FileHeader h{};
-        if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, MAGIC, 8) != 0 || h.version != 1)
+        if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, MAGIC, 8) != 0 || h.version != 2)
             throw std::invalid_argument("not a table file of this version");
         if(h.header_hash != hash_words(0x6B616368655F6864ull, &h, offsetof(FileHeader, header_hash)))
             throw std::invalid_argument("the table file's header is corrupt");
