@@ -2322,7 +2322,7 @@ int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* 
 int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, const uint64_t* bounds, uint64_t n_batches, uint32_t op,
                              uint64_t x, uint64_t* inserted) noexcept
 {
-    if(t == nullptr || !registered(t, user) || !device_text_ok(text, n_batches ? bounds[n_batches] : 0) || (n_batches > 0 && bounds == nullptr) ||
+    if(t == nullptr || !registered(t, user) || (n_batches > 0 && bounds == nullptr) || !device_text_ok(text, n_batches ? bounds[n_batches] : 0) ||
        (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
         return KACHE_EINVAL;
     for(uint64_t i = 0; i < n_batches; ++i)
@@ -2337,7 +2337,7 @@ int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, co
 
 int kache_amd_count_batches(const kache_table* t, const char* text, const uint64_t* bounds, uint64_t n_batches, uint64_t* hits, uint64_t* positions) noexcept
 {
-    if(t == nullptr || hits == nullptr || !device_text_ok(text, n_batches ? bounds[n_batches] : 0) || (n_batches > 0 && bounds == nullptr))
+    if(t == nullptr || hits == nullptr || (n_batches > 0 && bounds == nullptr) || !device_text_ok(text, n_batches ? bounds[n_batches] : 0))
         return KACHE_EINVAL;
     for(uint64_t i = 0; i < n_batches; ++i)
         if(bounds[i] > bounds[i + 1])

@@ -130,3 +130,12 @@ def test_tuning_validation(api):
         t.autotune(17)
     t.set_tuning(4, 20, True)
     assert t.tuning() == {"blocks_per_cu": 4, "launch_log2": 20, "graph": 1, "tuning": 0}
+    # Batch calls check their bounds before reading them.
+    u = t.register()
+    d = capi.DeviceBuffer(api, 64 + capi.TEXT_PAD)
+    L, E = api.lib, capi.KACHE_EINVAL
+    assert L.kache_amd_insert_batches(t.h, u, d.ptr, None, 3, capi.ADD, 1, None) == E
+    h = np.zeros(3, np.uint64)
+    assert L.kache_amd_count_batches(t.h, d.ptr, None, 3, h.ctypes.data, None) == E
+    bad = np.array([0, 32, 16], np.uint64)
+    assert L.kache_amd_insert_batches(t.h, u, d.ptr, bad.ctypes.data, 2, capi.ADD, 1, None) == E
