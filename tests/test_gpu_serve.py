@@ -217,7 +217,7 @@ def test_replay_open_loop(api):
         res = s.replay(made, arrivals)
     assert all(r["status"] == 0 for r in res)
     sub = np.array([r["submit_s"] for r in res]) - t0
-    assert (sub >= arrivals - 1e-4).all() and (sub <= arrivals + 0.05).all()
+    assert (sub >= arrivals - 1e-4).all() and (sub <= arrivals + 0.5).all()
     assert all(r["submit_s"] <= r["start_s"] <= r["done_s"] for r in res)
     assert sorted(r["ticket"] for r in res) == [r["ticket"] for r in res]
 
