@@ -185,6 +185,14 @@ __device__ __forceinline__ void min64(uint64_t* p, uint64_t v) { __hip_atomic_fe
 __device__ __forceinline__ void max64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void or32(uint32_t* p, uint32_t v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// Orders the wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Masks of the 4 bytes of x equal to the bytes of cc, as bits 0..3.
 __device__ __forceinline__ uint32_t byte_eq4(uint32_t x, uint32_t cc)
 {
@@ -473,18 +481,23 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             rem &= ~g;
         }
         const uint64_t below_me = (uint64_t(1) << __lane_id()) - 1;
-        // A lane whose key a lower lane of its group also has follows that lane.
+        // A lane whose key a lower lane of its group also has follows the
+        // lowest such lane.  Every lane takes part in each shuffle: a lane
+        // that had left a divergent loop would be read as inactive.
         int owner = -1;
-        if(need)
-            for(uint64_t m = grp & below_me; m; m &= m - 1)
+        uint64_t below = need ? grp & below_me : 0;
+        while(__ballot(below != 0))
+        {
+            const int l = below ? __builtin_ctzll(below) : static_cast<int>(__lane_id());
+            const uint64_t o = static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(sk), l, 64));
+            if(below)
             {
-                const int l = __builtin_ctzll(m);
-                if(static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(sk), l, 64)) == sk)
-                {
-                    owner = l;
-                    break;
-                }
+                if(o == sk)
+                    owner = l, below = 0;
+                else
+                    below &= below - 1;
             }
+        }
         const uint64_t leads = __ballot(need && owner < 0);
         if(need && owner < 0)
         {
@@ -498,6 +511,10 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             uint64_t occ = 0;
             if(j < B)
                 mine = cas64(slot_at<MAP>(bk, j), occ, rank == 0 ? sk : rk);
+            // Slots z .. j-1 were the targets of this group's lower-ranked
+            // lanes: those that won them hold other keys, so only a slot
+            // whose CAS lost can hold a concurrent claim of this key.
+            const uint64_t lost = __ballot(j < B && !mine);
             if(mine && rank == 0)
             {
                 bk[j] = c;
@@ -507,7 +524,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             {
                 // The lowest slot from z that holds or will hold this key.
                 uint32_t cand = B;
-                const uint32_t top = j < B ? j : B;
+                const uint32_t top = (grp & lost & below_me) ? (j < B ? j : B) : z;
                 for(uint32_t t = z; t < top; ++t)
                 {
                     const uint64_t v = ld_acq(slot_at<MAP>(bk, t));

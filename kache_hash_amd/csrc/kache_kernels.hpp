@@ -31,18 +31,46 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 // Inserts / upserts (include/Streaming_Kmer_Hash_Table.hpp:2217-2514): every
 // valid position of [pos_begin, pos_end) of `text`.  `x` is the operand, or
 // the forward one with ORIENT (x_rev for k-mers occurring reverse-complemented).
+//
+// Two paths per position.  The common one (a key already in its primary
+// bucket) reads the checksum line and the slots whose checksums match, then
+// applies the value atomic.  Every other position (a new key, or one placed
+// in a secondary bucket or the overflow table) is queued in the wave's LDS
+// ring and settled 64 at a time by insert_key_wave with the whole wave busy,
+// instead of holding a wave of mostly finished lanes through its claim rounds.  Operations within a
+// launch commute, so the order in which positions settle changes nothing.
+// A stale checksum or key line read by the common path only sends a
+// position to the queue; the queue's path reads slots coherently.
+constexpr uint32_t DEFER_RING = 128;   // > 63 left over + 64 new
+
+struct DeferRing
+{
+    uint64_t key[DEFER_RING];
+    uint64_t h[DEFER_RING];
+    uint32_t cf[DEFER_RING];   // checksum | forward << 8
+};
+
 template <bool MAP, uint32_t OP, bool ORIENT>
 __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
 {
     __shared__ TileLds s;
     __shared__ uint64_t red[TILE / 64];
+    __shared__ DeferRing rings[TILE / 64];
+    DeferRing& q = rings[threadIdx.x >> 6];
+    const uint32_t lane = __lane_id();
+    const uint64_t below_me = (uint64_t(1) << lane) - 1;
+    uint32_t head = 0, queued = 0;   // the wave's ring: entries [head, head + queued) mod DEFER_RING
     uint64_t ins = 0, ins_ov = 0, failed = 0;
-    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
+
+    // Settles the ring entry of each lane with `active` (all lanes take part).
+    auto settle = [&](bool active, uint32_t i)
     {
+        const uint64_t key = active ? q.key[i] : 0, h = active ? q.h[i] : 0;
+        const uint32_t cf = active ? q.cf[i] : 0;
         uint64_t* slot;
         bool in_ov;
-        const Res r = insert_key_wave<MAP>(T, d.valid, d.key, d.h, d.c, slot, in_ov);
-        if(!d.valid)
+        const Res r = insert_key_wave<MAP>(T, active, key, h, static_cast<uint8_t>(cf), slot, in_ov);
+        if(!active)
             return;
         if(r == RES_FULL)
             failed++;
@@ -50,9 +78,41 @@ __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restri
         {
             ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
             if constexpr(MAP)
-                value_op<OP>(slot + VAL_OFF, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
+                value_op<OP>(slot + VAL_OFF, (ORIENT && !(cf >> 8)) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
+        }
+    };
+
+    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
+    {
+        uint64_t* slot = nullptr;
+        if(d.valid)
+        {
+            bool full;
+            slot = probe_find<MAP>(bucket_at<MAP>(T, bucket_of(d.h, T.cap)), ~d.key, d.c, full);
+            if constexpr(MAP)
+                if(slot)
+                    value_op<OP>(slot + VAL_OFF, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, false);
+        }
+        const bool defer = d.valid && !slot;
+        const uint64_t m = __ballot(defer);
+        if(defer)
+        {
+            const uint32_t i = (head + queued + __popcll(m & below_me)) % DEFER_RING;
+            q.key[i] = d.key, q.h[i] = d.h, q.cf[i] = d.c | (d.fwd << 8);
+        }
+        queued += __popcll(m);
+        if(queued >= 64)
+        {
+            wave_sync();
+            settle(true, (head + lane) % DEFER_RING);
+            head = (head + 64) % DEFER_RING, queued -= 64;
         }
     });
+    if(queued)
+    {
+        wave_sync();
+        settle(lane < queued, (head + lane) % DEFER_RING);
+    }
     block_add(ins, &cnt->inserted, red);
     block_add(ins_ov, &cnt->ov_inserted, red);
     block_add(failed, &cnt->failed, red);
