@@ -26,7 +26,7 @@ constexpr uint32_t B = 32;                          // slots per bucket
 constexpr uint64_t KMER_MASK = (uint64_t(1) << (2 * K)) - 1;
 constexpr uint64_t LMER_MASK = (uint64_t(1) << (2 * L)) - 1;
 constexpr int TILE = 256;                           // positions per tile = threads per block
-constexpr int TILE_WORDS = 10;                      // 32-base words staged per tile: TILE + K + W - 1 bases
+constexpr int TILE_WORDS = 10;                      // 32-byte words readable from a tile start: its last wave stages bytes 192..319
 constexpr uint32_t META_BYTES = 64;
 constexpr uint64_t TEXT_PAD = 512;                  // readable non-ACGT bytes required past a text's end
 
@@ -140,9 +140,6 @@ __host__ __device__ __forceinline__ void window_parts(uint64_t w, uint64_t& kf, 
     lh = wang_u64(lf < lr ? lf : lr);
 }
 
-// 2-bit code of an ACGT character in either case (include/Kmer_Batch.hpp:108).
-__host__ __device__ __forceinline__ uint32_t code_of(uint32_t ch) { return ((ch >> 2) ^ (ch >> 1)) & 3u; }
-
 // Whether a character is ACGT in either case (include/DNA_Utility.hpp:199-224).
 __host__ __device__ __forceinline__ uint32_t is_base(uint32_t ch)
 {
@@ -150,21 +147,31 @@ __host__ __device__ __forceinline__ uint32_t is_base(uint32_t ch)
     return (x == 'a') | (x == 'c') | (x == 'g') | (x == 't');
 }
 
-// Packs 32 characters (8 little-endian words) into their 2-bit codes, first
-// character most significant, and a mask of the non-ACGT ones (bit j: char j).
-__host__ __device__ __forceinline__ void pack32(const uint32_t (&d)[8], uint64_t& pk, uint32_t& bad)
+// Packs 8 characters (char j in byte j of v) into their 2-bit codes
+// ((ch >> 2) ^ (ch >> 1)) & 3, A/a 0, C/c 1, G/g 2, T/t 3
+// (include/Kmer_Batch.hpp:108), first character most significant (16 bits),
+// and a mask of the non-ACGT ones (bit j: char j; is_base), with the bytes
+// processed side by side in the 64-bit word.
+__host__ __device__ __forceinline__ void pack8(uint64_t v, uint32_t& code, uint32_t& bad)
 {
-    pk = 0, bad = 0;
-#pragma unroll
-    for(int i = 0; i < 8; ++i)
-#pragma unroll
-        for(int b = 0; b < 4; ++b)
-        {
-            const uint32_t ch = (d[i] >> (8 * b)) & 0xFFu;
-            const int j = 4 * i + b;
-            pk |= static_cast<uint64_t>(code_of(ch)) << (2 * (31 - j));
-            bad |= (is_base(ch) ^ 1u) << j;
-        }
+    // Codes of the byte-reversed word, char j's code at bits 8 (7 - j), then
+    // gathered pairwise: char j ends at bits 2 (7 - j).
+    uint64_t c = __builtin_bswap64(v);
+    c = ((c >> 2) ^ (c >> 1)) & 0x0303030303030303ull;
+    c = (c | (c >> 6)) & 0x000F000F000F000Full;
+    c = (c | (c >> 12)) & 0x000000FF000000FFull;
+    code = static_cast<uint32_t>((c | (c >> 24)) & 0xFFFFu);
+    // Bytes equal to a, c, g or t after case folding (an exact zero-byte test
+    // per candidate), then the high bit of each non-base byte gathered to bit j.
+    const uint64_t x = v | 0x2020202020202020ull, lo7 = 0x7F7F7F7F7F7F7F7Full;
+    uint64_t good = 0;
+    for(const uint64_t b : {0x61ull, 0x63ull, 0x67ull, 0x74ull})
+    {
+        const uint64_t y = x ^ (b * 0x0101010101010101ull);
+        good |= ~(((y & lo7) + lo7) | y | lo7);
+    }
+    const uint64_t nb = (~good & 0x8080808080808080ull) >> 7;
+    bad = static_cast<uint32_t>((nb * 0x0102040810204080ull) >> 56);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,12 +683,15 @@ __device__ __forceinline__ void value_op(uint64_t* v, uint64_t x, uint64_t vmax,
 }
 
 // ---------------------------------------------------------------------------
-// The describer, per tile of TILE positions: each thread describes position
-// p0 + threadIdx.x.  Staging: 10 threads pack 320 characters into 2-bit
-// words and non-ACGT masks in LDS; every thread then builds its 32-base
-// window from two words, hashes its l-mer, and takes the minimum of 10
-// consecutive l-mer hashes from LDS (include/Kmer_Batch.hpp:191-250, the
-// same quantities per position).
+// The describer.  Positions go in tiles of TILE (one per block and turn);
+// wave w of a block describes positions 64w .. 64w + 63 of the tile on its
+// own, with no block barrier, so a wave waiting on its probes holds back no
+// other wave.  Staging: 16 lanes pack 8 characters each (the wave's
+// 64 + K + W - 2 bases, rounded up to 128) into 2-bit codes and non-ACGT
+// masks in the wave's LDS words; every lane then builds its 32-base window
+// from two words, hashes its l-mer, and takes the minimum of 10 consecutive
+// l-mer hashes from LDS (include/Kmer_Batch.hpp:191-250, the same quantities
+// per position).
 
 struct Desc
 {
@@ -692,58 +702,53 @@ struct Desc
     uint8_t c;
 };
 
-struct TileLds
+constexpr int WAVE_WORDS = 4;           // 32-base words a wave stages
+constexpr int PACK_LANES = 8 * WAVE_WORDS / 2;   // lanes packing 8 characters each
+
+struct WaveLds
 {
-    uint64_t pk[TILE_WORDS];
-    uint32_t bad[TILE_WORDS];
-    uint64_t lh[TILE + W];
+    uint64_t pk[WAVE_WORDS];            // 2-bit codes, first base most significant
+    uint32_t bad[WAVE_WORDS];           // bit j: character j of the word is not ACGT
+    uint64_t lh[64 + W];
 };
 
-__device__ __forceinline__ uint64_t window_at(const TileLds& s, int u)
+__device__ __forceinline__ uint64_t window_at(const WaveLds& s, int u)
 {
     const int wi = u >> 5, sh = (u & 31) * 2;
     const uint64_t a = s.pk[wi];
     return sh ? (a << sh) | (s.pk[wi + 1] >> (64 - sh)) : a;
 }
 
-// `text + p0` must be 16-byte aligned with TILE_WORDS * 32 readable bytes.
-// The 32-byte text words a tile's first TILE_WORDS threads stage, loaded a
-// tile ahead so their latency hides behind the previous tile's probes.
-struct TileWords
+// Starts loading the 8 characters a packing lane stages for the tile at p0:
+// bytes p0 + 64w + 8i .. + 7 for lane i of wave w, within the TILE_WORDS * 32
+// readable bytes that `text + p0` (16-byte aligned) must have.
+__device__ __forceinline__ void load_wave_tile(const uint8_t* text, uint64_t p0, uint2& w)
 {
-    uint4 a, b;
-};
-
-// Starts loading the words of the tile at p0 (threads below TILE_WORDS).
-// `text + p0` must be 16-byte aligned with TILE_WORDS * 32 readable bytes.
-__device__ __forceinline__ void load_tile(const uint8_t* text, uint64_t p0, TileWords& w)
-{
-    if(threadIdx.x < TILE_WORDS)
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(text + p0 + 32 * threadIdx.x);
-        w.a = src[0], w.b = src[1];
-    }
+    const uint32_t lane = __lane_id();
+    if(lane < PACK_LANES)
+        w = *reinterpret_cast<const uint2*>(text + p0 + 64 * (threadIdx.x >> 6) + 8 * lane);
 }
 
-// Describes the tile at p0 from its loaded words `w`, then starts loading
-// the words of the tile at `p_next` into `w` (if `has_next`).  Positions at
-// or past `pos_end` are not described; a k-mer must also end within the
-// first `text_len` bytes.
-__device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, uint64_t pos_end, uint64_t text_len, TileLds& s,
-                                              TileWords& w, uint64_t p_next, bool has_next)
+// Describes position p0 + threadIdx.x of the tile at p0 from the wave's
+// loaded characters `w`, then starts loading those of the tile at `p_next`
+// into `w` (if `has_next`).  Positions at or past `pos_end` are not
+// described; a k-mer must also end within the first `text_len` bytes.
+__device__ __forceinline__ Desc describe_wave(const uint8_t* text, uint64_t p0, uint64_t pos_end, uint64_t text_len, WaveLds& s,
+                                              uint2& w, uint64_t p_next, bool has_next)
 {
-    const int t = threadIdx.x;
-    if(t < TILE_WORDS)
+    const int t = static_cast<int>(__lane_id());
+    if(t < PACK_LANES)
     {
-        const uint32_t d[8] = {w.a.x, w.a.y, w.a.z, w.a.w, w.b.x, w.b.y, w.b.z, w.b.w};
-        uint64_t pk;
-        uint32_t bad;
-        pack32(d, pk, bad);
-        s.pk[t] = pk, s.bad[t] = bad;
+        // Lane t holds characters 8t .. 8t + 7: bits 16 (3 - t % 4) of word
+        // t / 4 (its first character most significant) and byte t % 4 of its mask.
+        uint32_t code, bad;
+        pack8(static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32), code, bad);
+        reinterpret_cast<uint16_t*>(s.pk)[(t & ~3) + 3 - (t & 3)] = static_cast<uint16_t>(code);
+        reinterpret_cast<uint8_t*>(s.bad)[t] = static_cast<uint8_t>(bad);
     }
-    __syncthreads();
+    wave_sync();
     if(has_next)
-        load_tile(text, p_next, w);
+        load_wave_tile(text, p_next, w);
 
     Desc d;
     uint64_t kf, kr, lh;
@@ -752,14 +757,14 @@ __device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, 
     if(t < W - 1)
     {
         uint64_t f2, r2, lh2;
-        window_parts(window_at(s, TILE + t), f2, r2, lh2);
-        s.lh[TILE + t] = lh2;
+        window_parts(window_at(s, 64 + t), f2, r2, lh2);
+        s.lh[64 + t] = lh2;
     }
-
     const int wi = t >> 5;
     const uint64_t b64 = static_cast<uint64_t>(s.bad[wi]) | (static_cast<uint64_t>(s.bad[wi + 1]) << 32);
-    d.valid = (((b64 >> (t & 31)) & 0x7FFFFFFFull) == 0) && (p0 + t < pos_end) && (p0 + t + K <= text_len);
-    __syncthreads();
+    const uint64_t p = p0 + threadIdx.x;
+    d.valid = (((b64 >> (t & 31)) & 0x7FFFFFFFull) == 0) && (p < pos_end) && (p + K <= text_len);
+    wave_sync();
 
     uint64_t h = s.lh[t];
 #pragma unroll
@@ -772,26 +777,29 @@ __device__ __forceinline__ Desc describe_tile(const uint8_t* text, uint64_t p0, 
     d.key = kf < kr ? kf : kr;
     d.fwd = kf <= kr;
     d.c = checksum(d.key);
+    // The next tile's codes overwrite the LDS words only after every lane has read them.
+    wave_sync();
     return d;
 }
 
 // Walks the positions [p_begin, pos_end) in tiles, block by block, each
-// tile's text prefetched during the previous one: calls f(tile, desc), where
-// tile t covers positions origin + t * TILE onwards and origin is p_begin
-// rounded down to 16 bytes (tiles load aligned words); positions before
-// p_begin are not described.
+// wave's words prefetched during its previous tile: calls f(tile, desc) on
+// every lane, where tile t covers positions origin + t * TILE onwards (lane
+// position origin + t * TILE + threadIdx.x) and origin is p_begin rounded
+// down to 16 bytes (tiles load aligned words); positions before p_begin are
+// not described.  `s` is the wave's own LDS.
 template <typename F>
-__device__ __forceinline__ void for_tiles(const uint8_t* text, uint64_t p_begin, uint64_t pos_end, uint64_t text_len, TileLds& s, const F& f)
+__device__ __forceinline__ void for_tiles(const uint8_t* text, uint64_t p_begin, uint64_t pos_end, uint64_t text_len, WaveLds& s, const F& f)
 {
     const uint64_t origin = p_begin & ~uint64_t(15);
     const uint64_t tiles = (pos_end - origin + TILE - 1) / TILE;
-    TileWords w{};
+    uint2 w{};
     if(blockIdx.x < tiles)
-        load_tile(text, origin + blockIdx.x * uint64_t(TILE), w);
+        load_wave_tile(text, origin + blockIdx.x * uint64_t(TILE), w);
     for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
         const uint64_t next = tile + gridDim.x;
-        Desc d = describe_tile(text, origin + tile * TILE, pos_end, text_len, s, w, origin + next * TILE, next < tiles);
+        Desc d = describe_wave(text, origin + tile * TILE, pos_end, text_len, s, w, origin + next * TILE, next < tiles);
         d.valid &= (origin + tile * TILE + threadIdx.x >= p_begin);
         f(tile, d);
     }

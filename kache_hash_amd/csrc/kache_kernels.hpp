@@ -53,9 +53,10 @@ struct DeferRing
 template <bool MAP, uint32_t OP, bool ORIENT>
 __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
 {
-    __shared__ TileLds s;
+    __shared__ WaveLds wave_lds[TILE / 64];
     __shared__ uint64_t red[TILE / 64];
     __shared__ DeferRing rings[TILE / 64];
+    WaveLds& s = wave_lds[threadIdx.x >> 6];
     DeferRing& q = rings[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
     const uint64_t below_me = (uint64_t(1) << lane) - 1;
@@ -122,7 +123,8 @@ __global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restri
 template <uint32_t OP>
 __global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, Counters* cnt)
 {
-    __shared__ TileLds s;
+    __shared__ WaveLds wave_lds[TILE / 64];
+    WaveLds& s = wave_lds[threadIdx.x >> 6];
     __shared__ uint64_t red[TILE / 64];
     uint64_t present = 0;
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
@@ -177,7 +179,8 @@ __device__ __forceinline__ uint32_t segment_of(const QueryArgs& a, uint64_t tile
 template <bool MAP, int MODE>
 __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_begin, uint64_t pos_end, QueryArgs a, Counters* cnt)
 {
-    __shared__ TileLds s;
+    __shared__ WaveLds wave_lds[TILE / 64];
+    WaveLds& s = wave_lds[threadIdx.x >> 6];
     __shared__ uint64_t red[TILE / 64];
     uint64_t total = 0, positions = 0;
     for_tiles(text, pos_begin, pos_end, pos_end, s, [&](uint64_t tile, const Desc& d)
@@ -655,11 +658,24 @@ __global__ void k_zero64(uint64_t* p, uint64_t n)
         p[i] = 0;
 }
 
+// Moves n u64 words from `src` (device) to `dst` (host memory) and zeroes
+// `src`: a launch's counters read back and cleared for the next launch by
+// one dispatch.
+__global__ void k_publish(uint64_t* src, uint64_t* dst, uint64_t n)
+{
+    for(uint64_t i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        src[i] = 0;
+    }
+}
+
 // Describes the valid positions of [pos_begin, pos_end) into arrays (for
 // tests: the device describer checked position by position).
 __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid)
 {
-    __shared__ TileLds s;
+    __shared__ WaveLds wave_lds[TILE / 64];
+    WaveLds& s = wave_lds[threadIdx.x >> 6];
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t tile, const Desc& d)
     {
         const uint64_t p = (pos_begin & ~uint64_t(15)) + tile * TILE + threadIdx.x;

@@ -530,15 +530,22 @@ struct kache_table
         HIP(hipGetLastError());
     }
 
+    // Enqueues the read-back of the counters of the launches so far into
+    // `out` (host memory) and their zeroing for the next launches.
+    void publish_counters(Counters* dev, Counters* out) const
+    {
+        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(dev), reinterpret_cast<uint64_t*>(out),
+                           sizeof(Counters) / 8);
+        HIP(hipGetLastError());
+    }
+
     // Reads the counters of the launches since the last read, and zeroes them
     // again for the next (they are zero whenever no launch is in flight).
     Counters read_counters() const
     {
-        HIP(hipMemcpyAsync(h_cnt, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+        publish_counters(d_cnt, h_cnt);
         sync();
-        const Counters c = *h_cnt;
-        zero_counters();
-        return c;
+        return *h_cnt;
     }
 
     // --- Overflow growth (include/Streaming_Kmer_Hash_Table.hpp:1412-1443) ---
@@ -861,7 +868,7 @@ struct kache_table
     kache_amd_tuning tuned_from{};
     size_t tune_next = 0;
 
-    // One graph per insert kernel: [kernel, counters to host, zero counters].
+    // One graph per insert kernel: [kernel, counters to host and zeroed].
     struct InsertGraph
     {
         hipGraph_t graph = nullptr;
@@ -895,10 +902,9 @@ struct kache_table
             HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
             hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(TILE), a.ptrs, 0, stream);
             if(le == hipSuccess)
-                le = hipMemcpyAsync(h_cnt, d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream);
-            if(le == hipSuccess)
             {
-                hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), sizeof(Counters) / 8);
+                hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), reinterpret_cast<uint64_t*>(h_cnt),
+                                   sizeof(Counters) / 8);
                 le = hipGetLastError();
             }
             hipGraph_t graph = nullptr;
@@ -1248,9 +1254,7 @@ struct kache_table
             }
             const int slot = static_cast<int>(next_slot++ % RING);
             timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot); });
-            HIP(hipMemcpyAsync(h_ring + slot, d_ring + slot, sizeof(Counters), hipMemcpyDeviceToHost, stream));
-            hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_ring + slot), sizeof(Counters) / 8);
-            HIP(hipGetLastError());
+            publish_counters(d_ring + slot, h_ring + slot);
             HIP(hipEventRecord(ring_ev[slot], stream));
             q.push_back({i, slot, pos});
             in_flight += pos;
@@ -2510,26 +2514,29 @@ int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key
 {
     if(text_bytes > 0 && (!text || !key || !h || !c || !fwd || !valid))
         return KACHE_EINVAL;
-    // The device tile's steps, on the host: 32-character words packed with
-    // pack32, 32-base windows from two words, l-mer hashes, window minima.
+    // The device describer's steps, on the host: 32-base words packed 8
+    // characters at a time with pack8, 32-base windows from two words, l-mer
+    // hashes, window minima.
     const uint64_t words = (text_bytes + TILE_WORDS * 32) / 32 + 2;
     std::vector<uint64_t> pk(words, 0);
     std::vector<uint32_t> bad(words, ~0u);
     for(uint64_t w = 0; w < words; ++w)
     {
-        uint32_t d[8];
-        for(int i = 0; i < 8; ++i)
+        pk[w] = 0, bad[w] = 0;
+        for(int q = 0; q < 4; ++q)
         {
-            uint32_t v = 0;
-            for(int b = 0; b < 4; ++b)
+            uint64_t v = 0;
+            for(int b = 0; b < 8; ++b)
             {
-                const uint64_t at = w * 32 + 4 * i + b;
-                const uint32_t ch = at < text_bytes ? static_cast<uint8_t>(text[at]) : '\n';
+                const uint64_t at = w * 32 + 8 * q + b;
+                const uint64_t ch = at < text_bytes ? static_cast<uint8_t>(text[at]) : '\n';
                 v |= ch << (8 * b);
             }
-            d[i] = v;
+            uint32_t code, nb;
+            pack8(v, code, nb);
+            pk[w] |= static_cast<uint64_t>(code) << (16 * (3 - q));
+            bad[w] |= nb << (8 * q);
         }
-        pack32(d, pk[w], bad[w]);
     }
     const auto window = [&](uint64_t u)
     {
