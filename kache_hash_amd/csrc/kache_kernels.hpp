@@ -260,33 +260,50 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
     const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     const uint64_t pairs = (O.cap + 1) / 2;
-    for(uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; w < pairs; w += (uint64_t(gridDim.x) * blockDim.x) >> 6)
+    const uint64_t stride = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    // Slot j of old bucket 2w + lane / 32: its checksum, word and value, all
+    // read at once (an empty slot's words are 0), a pair ahead of its use.
+    struct Old
     {
+        uint64_t sk, val;
+        uint32_t c;
+    };
+    const auto read = [&](uint64_t w)
+    {
+        Old o{0, 0, 0};
+        const uint64_t b = 2 * w + (lane >> 5);
+        if(w < pairs && b < O.cap)
+        {
+            const uint8_t* ob = bucket_at<MAP>(O, b);
+            const uint64_t* os = slot_at<MAP>(const_cast<uint8_t*>(ob), j);
+            o.c = ob[j], o.sk = os[0];
+            if(MAP)
+                o.val = os[VAL_OFF];
+        }
+        return o;
+    };
+    uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6;
+    Old next = read(w);
+    for(; w < pairs; w += stride)
+    {
+        const Old cur = next;
+        next = read(w + stride);
         const uint64_t b = 2 * w + (lane >> 5);
         bool lo = false, hi = false, aw = false;
-        uint64_t sk = 0, val = 0;
-        uint8_t c = 0;
-        if(b < O.cap)
+        const uint64_t sk = cur.sk, val = cur.val;
+        uint8_t c = static_cast<uint8_t>(cur.c);
+        if(c && !is_key(sk))
+            c = 0;      // a tombstone: dropped
+        if(c)
         {
-            uint8_t* ob = bucket_at<MAP>(O, b);
-            c = ob[j];
-            const uint64_t* os = slot_at<MAP>(ob, j);
-            if(c && !is_key(os[0]))
-                c = 0;      // a tombstone: dropped
-            if(c)
+            const uint64_t h = minimizer_hash(~sk);
+            if(bucket_of(h, O.cap) == b)
             {
-                sk = os[0];
-                if(MAP)
-                    val = os[VAL_OFF];
-                const uint64_t h = minimizer_hash(~sk);
-                if(bucket_of(h, O.cap) == b)
-                {
-                    hi = (bucket_of(h, N.cap) == 2 * b + 1);
-                    lo = !hi;
-                }
-                else
-                    aw = true;
+                hi = (bucket_of(h, N.cap) == 2 * b + 1);
+                lo = !hi;
             }
+            else
+                aw = true;
         }
         const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask, m_away = __ballot(aw);
         if(b < O.cap)
