@@ -5,16 +5,18 @@
 
 #include "kache_device.hpp"
 
-// Occupancy of the streaming kernels (insert, update, query): minimum waves
-// per SIMD the compiler must fit, which caps their registers.
+// Occupancy of the streaming kernels: minimum waves per SIMD the compiler
+// must fit, which caps their registers.  The insert kernel's settle path
+// needs more registers than the lookups: at 8 waves its common path reloads
+// spilled values inside the loop; 6 waves with fewer spills measured faster.
 #ifndef KAMD_STREAM_WAVES
 #define KAMD_STREAM_WAVES 8
 #endif
-#if KAMD_STREAM_WAVES > 0
-#define KAMD_STREAM_BOUNDS __launch_bounds__(256, KAMD_STREAM_WAVES)
-#else
-#define KAMD_STREAM_BOUNDS __launch_bounds__(256)
+#ifndef KAMD_INSERT_WAVES
+#define KAMD_INSERT_WAVES 6
 #endif
+#define KAMD_STREAM_BOUNDS __launch_bounds__(256, KAMD_STREAM_WAVES)
+#define KAMD_INSERT_BOUNDS __launch_bounds__(256, KAMD_INSERT_WAVES)
 
 namespace kamd
 {
@@ -51,7 +53,7 @@ struct DeferRing
 };
 
 template <bool MAP, uint32_t OP, bool ORIENT>
-__global__ KAMD_STREAM_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
+__global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
 {
     __shared__ WaveLds wave_lds[TILE / 64];
     __shared__ uint64_t red[TILE / 64];
