@@ -2,6 +2,8 @@
 // reference's tools/memory-roof.cpp (phases :222-354): over an array as large
 // as the tables it bounds,
 //   stream          sequential 16-byte reads of every line (GB/s)
+//   copy, write     a sequential copy of half the array onto the other half
+//                   (bytes read + written), sequential 16-byte writes (GB/s)
 //   random_read     independent reads of one word of random lines (lines/s)
 //   random_update   independent read-modify-writes of one word of random lines
 //   random_atomic   independent 64-bit atomic adds to random lines, no return
@@ -59,6 +61,22 @@ __global__ void k_stream(const uint4* a, uint64_t n16, uint64_t* sink)
     }
     if(s == 0x12345678u)
         atomicAdd(reinterpret_cast<unsigned long long*>(sink), 1ull);
+}
+
+// Copies n16 16-byte words from src to dst (streams both ways).
+__global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+// Writes n16 16-byte words.
+__global__ void k_write(uint4* a, uint64_t n16)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n16; i += stride)
+        a[i] = make_uint4(uint32_t(i), 0u, 0u, 0u);
 }
 
 // MODE 0 read, 1 read-modify-write, 2 atomic add, 3 atomic add with return,
@@ -186,6 +204,20 @@ int main(int argc, char** argv)
             timed([&]{ hipLaunchKernelGGL(k_stream, dim3(grid), dim3(block), 0, 0, reinterpret_cast<const uint4*>(a), bytes / 16, sink); });
             const double t = timed([&]{ hipLaunchKernelGGL(k_stream, dim3(grid), dim3(block), 0, 0, reinterpret_cast<const uint4*>(a), bytes / 16, sink); });
             std::printf(", \"stream_GBps\": %.1f", bytes / t / 1e9);
+            // Copy of one half onto the other (bytes read + bytes written), then a
+            // stream of writes over the whole array; the array is set up again after.
+            const uint64_t half16 = bytes / 32;
+            const auto copy = [&]{ hipLaunchKernelGGL(k_copy, dim3(grid), dim3(block), 0, 0, reinterpret_cast<const uint4*>(a),
+                                                      reinterpret_cast<uint4*>(a) + half16, half16); };
+            timed(copy);
+            const double tcp = timed(copy);
+            std::printf(", \"copy_GBps\": %.1f", 2.0 * half16 * 16 / tcp / 1e9);
+            const auto write = [&]{ hipLaunchKernelGGL(k_write, dim3(grid), dim3(block), 0, 0, reinterpret_cast<uint4*>(a), bytes / 16); };
+            timed(write);
+            const double tw = timed(write);
+            std::printf(", \"write_GBps\": %.1f", bytes / tw / 1e9);
+            hipLaunchKernelGGL(k_init, dim3(grid), dim3(block), 0, 0, a, words, lw, lines);
+            CHECK(hipDeviceSynchronize());
         }
         const uint32_t touch = line == 128 ? 2 : 1;     // words read per line: both 64-byte halves of a 128-byte line
         const double n = double(lanes) * iters;
