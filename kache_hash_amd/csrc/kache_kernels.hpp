@@ -258,6 +258,8 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
 template <bool MAP>
 __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64_t* away)
 {
+    constexpr uint32_t BB = Geo<MAP>::bucket_bytes, IMG = 2 * BB;
+    __shared__ __align__(16) uint8_t images[TILE / 64][2][IMG];
     const uint32_t lane = threadIdx.x & 63, j = lane & 31;
     const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
     const uint64_t below = (uint64_t(1) << lane) - 1;
@@ -308,45 +310,31 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
                 aw = true;
         }
         const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask, m_away = __ballot(aw);
+        // The half's two new buckets (2b, 2b + 1, adjacent in memory) are
+        // assembled in LDS, zeroed, with each moving key at its rank among
+        // those going to the same bucket, then stored whole in 16-byte words;
+        // the new buckets need no zeroing beforehand.
+        uint8_t* img = images[threadIdx.x >> 6][lane >> 5];
+        wave_sync();    // the previous pair's image has been read out
+        for(uint32_t k = j; k < IMG / 16; k += 32)
+            reinterpret_cast<uint4*>(img)[k] = make_uint4(0, 0, 0, 0);
+        wave_sync();
+        if(lo || hi)
+        {
+            const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
+            uint8_t* nb = img + (lo ? 0 : BB);
+            uint64_t* ns = reinterpret_cast<uint64_t*>(nb + META_BYTES) + r;
+            ns[0] = sk;
+            if(MAP)
+                ns[VAL_OFF] = val;
+            nb[r] = c;
+        }
+        wave_sync();
         if(b < O.cap)
         {
-            // Lane j of the half writes slot j of both new buckets: the key
-            // ranked j among those moving there, or an empty slot; the new
-            // buckets need no zeroing beforehand.
-            const uint32_t n_lo = __popcll(m_lo), n_hi = __popcll(m_hi);
-            uint8_t* nlo = bucket_at<MAP>(N, 2 * b);
-            uint8_t* nhi = nlo + Geo<MAP>::bucket_bytes;
-            if(lo || hi)
-            {
-                const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
-                uint8_t* nb = lo ? nlo : nhi;
-                uint64_t* ns = slot_at<MAP>(nb, r);
-                ns[0] = sk;
-                if(MAP)
-                    ns[VAL_OFF] = val;
-                nb[r] = c;
-            }
-            if(j >= n_lo)
-            {
-                uint64_t* ns = slot_at<MAP>(nlo, j);
-                ns[0] = 0;
-                if(MAP)
-                    ns[VAL_OFF] = 0;
-                nlo[j] = 0;
-            }
-            if(j >= n_hi)
-            {
-                uint64_t* ns = slot_at<MAP>(nhi, j);
-                ns[0] = 0;
-                if(MAP)
-                    ns[VAL_OFF] = 0;
-                nhi[j] = 0;
-            }
-            if(j < 8)   // the spill filters start clear
-            {
-                reinterpret_cast<uint32_t*>(nlo + 32)[j] = 0;
-                reinterpret_cast<uint32_t*>(nhi + 32)[j] = 0;
-            }
+            uint4* dst = reinterpret_cast<uint4*>(bucket_at<MAP>(N, 2 * b));
+            for(uint32_t k = j; k < IMG / 16; k += 32)
+                dst[k] = reinterpret_cast<const uint4*>(img)[k];
         }
         if(lane == 0)
             away[w] = m_away;
