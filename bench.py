@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--served-requests", type=int, default=200,
+                   help="count-map: requests of the served-latency probe on the built table (0: none)")
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--seed", type=int, default=None)
     p.add_argument("--workload", choices=["count-map", "serve", "counter", "batched"], default="count-map")
@@ -241,6 +243,8 @@ def main():
         except Exception:
             traffic = None
 
+    served = served_probe(a, KA, capi, api, t, u) if a.served_requests > 0 else None
+
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -270,10 +274,46 @@ def main():
                      "kmers_per_launch": per_launch_kmers, "avg_launch_ms": avg_ms,
                      "random_line_roof": random_line_roof(per_launch_kmers / (avg_ms * 1e-3) if avg_ms > 0 else 0.0)},
         "cpu_baseline": cpu,
+        "served": served,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def served_probe(a, KA, capi, api, t, u):
+    """Served-batch latency on the count map just built: requests of the
+    serving workload's mix and shapes drawn from the same genome (count,
+    lookup, update +1, upsert +1), run by the serving loop and by plain
+    per-request calls, closed loop and at Poisson arrivals of half the plain
+    path's capacity.  After the timed region; its upserts change the table."""
+    g = capi.DeviceBuffer(api, a.genome)
+    api.check(api.lib.kache_amd_synth_genome(g.ptr, a.genome, a.seed), "synth_genome")
+    genome = g.download(a.genome)
+    g.free()
+    reqs = serve_requests(capi, genome, a.served_requests, a.seed + 5, a.genome)
+    made = [capi.make_request(8, k, b, capi.ADD, 1) for k, b in reqs]
+    for m in made:
+        for arr in m[1].values():
+            arr.fill(1)
+    WAVE = 64 << 20
+    serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)     # warm-up
+    cap_s = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)
+    cap_p = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1)
+    rate = 0.5 * cap_p["requests_per_s"]
+    arr = np.cumsum(np.random.default_rng(a.seed + 6).exponential(1.0 / rate, len(reqs)))
+    at_s = serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20)
+    at_p = serve_run(capi, t, u, reqs, made, arr, WAVE, 1)
+    print(f"bench: served {cap_s['requests_per_s']:.0f} req/s vs plain {cap_p['requests_per_s']:.0f} req/s; at {rate:.0f} req/s "
+          f"served p50 {at_s['p50_ms']:.2f} p99 {at_s['p99_ms']:.2f} ms, plain p50 {at_p['p50_ms']:.2f} p99 {at_p['p99_ms']:.2f} ms",
+          file=sys.stderr, flush=True)
+    return {"requests": len(reqs), "kmer_positions": sum(b.positions() for _, b in reqs),
+            "mix": "35% count, 35% lookup, 5% update +1, 25% upsert +1; 1-10,000 sequences, lognormal median 300 bp",
+            "closed_loop": {"served_req_per_s": cap_s["requests_per_s"], "served_kmers_per_s": cap_s["kmers_per_s"],
+                            "plain_req_per_s": cap_p["requests_per_s"], "plain_kmers_per_s": cap_p["kmers_per_s"],
+                            "served_p50_ms": cap_s["p50_ms"], "served_p99_ms": cap_s["p99_ms"]},
+            "poisson": {"rate_req_per_s": rate, "served_p50_ms": at_s["p50_ms"], "served_p99_ms": at_s["p99_ms"],
+                        "plain_p50_ms": at_p["p50_ms"], "plain_p99_ms": at_p["p99_ms"]}}
 
 
 # --- Serving workload ------------------------------------------------------
