@@ -603,6 +603,92 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
     return r;
 }
 
+// insert_key_wave for keys known to be absent from the table and distinct
+// from every other key being inserted concurrently: a doubling's reinsert,
+// whose keys each sit once in the old table and are neither among the
+// primaries the split moved nor among one another.  Then no claim can meet
+// its own key, so the lanes of a group take their slots z, z + 1, ... with
+// their keys at once: no checksum matches to read, no reservations to turn
+// into keys, no slots below to scan.  A lost CAS means the slot went to
+// another key: the lane tries again above it.  Same policy, same result
+// (RES_NEW, or RES_FULL if the overflow table has no room).
+template <bool MAP>
+__device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
+{
+    const uint64_t sk = ~key;
+    in_ov = false;
+    slot = nullptr;
+    Res r = RES_NEW;
+    uint32_t stage = 0, lo = 0;
+    uint64_t bidx = active ? bucket_of(h, T.cap) : 0;
+    bool need = active;
+    const uint64_t below_me = (uint64_t(1) << __lane_id()) - 1;
+    for(uint32_t round = 0; __ballot(need); ++round)
+    {
+        if(round == CLAIM_ROUNDS)
+        {
+            if(need)
+                r = RES_FULL, need = false;
+            break;
+        }
+        uint8_t* bk = bucket_at<MAP>(T, bidx);
+        uint32_t z = B;
+        if(need)
+        {
+            uint32_t eq, zero;
+            cs_masks(bk, c, eq, zero);
+            z = zero ? __builtin_ctz(zero) : B;
+            z = z > lo ? z : lo;
+        }
+        const uint64_t claimers = __ballot(need);
+        const uint64_t gkey = (bidx << 6) | z;
+        uint64_t grp = 0;
+        for(uint64_t rem = claimers; rem;)
+        {
+            const uint64_t gl = readlane64(gkey, __builtin_ctzll(rem));
+            const uint64_t g = __ballot(need && gkey == gl);
+            if(need && gkey == gl)
+                grp = g;
+            rem &= ~g;
+        }
+        if(need)
+        {
+            const uint32_t j = z + __popcll(grp & below_me);
+            if(j < B)
+            {
+                uint64_t occ = 0;
+                if(cas64(slot_at<MAP>(bk, j), occ, sk))
+                {
+                    bk[j] = c;
+                    slot = slot_at<MAP>(bk, j), need = false;
+                }
+                else
+                    lo = j + 1;
+            }
+            else
+            {
+                // Every slot is taken: the next bucket of the policy.
+                if(stage == 0)
+                    mark_spill(bk, h);
+                if(stage < 2)
+                {
+                    uint64_t p, q;
+                    secondaries(h, T.cap, p, q);
+                    bidx = stage == 0 ? p : q;
+                    stage++, lo = 0;
+                }
+                else
+                {
+                    in_ov = true;
+                    r = ov_insert<MAP>(T, key, sk, slot);
+                    need = false;
+                }
+            }
+        }
+    }
+    return r;
+}
+
 // Lookup policy (include/Streaming_Kmer_Hash_Table.hpp:2548-2574): the
 // primary; absent unless it is full and its filter marks the minimizer; the
 // secondaries; the overflow table iff both are full.

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                 uint64_t* slot;
                 bool in_ov;
                 const uint64_t key = ~sk;
-                const Res r = insert_key_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+                const Res r = insert_new_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
                 if(active)
                 {
                     if(r == RES_FULL)
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
             uint64_t* slot;
             bool in_ov;
             const uint64_t key = ~sk;
-            const Res r = insert_key_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+            const Res r = insert_new_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
             if(active)
             {
                 if(r == RES_FULL)
