@@ -530,12 +530,12 @@ struct kache_table
         HIP(hipGetLastError());
     }
 
-    // Enqueues the read-back of the counters of the launches so far into
+    // Enqueues the read-back of `n` counter sets of the launches so far into
     // `out` (host memory) and their zeroing for the next launches.
-    void publish_counters(Counters* dev, Counters* out) const
+    void publish_counters(Counters* dev, Counters* out, uint64_t n = 1) const
     {
         hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(dev), reinterpret_cast<uint64_t*>(out),
-                           sizeof(Counters) / 8);
+                           n * sizeof(Counters) / 8);
         HIP(hipGetLastError());
     }
 
@@ -1190,20 +1190,39 @@ struct kache_table
     // in flight cannot reach the growth threshold nor fill the overflow
     // table: then no doubling can fall due among them, and the growth
     // schedule, applied batch by batch as their counts arrive, is exact.
+    // Each batch counts into its own ring slot; the slots of up to
+    // PUBLISH_GROUP consecutive batches are copied to host memory by one
+    // kernel, earlier when the next batch could not be launched before a
+    // count is read.
     uint64_t insert_batches(uint32_t user, const uint8_t* text, const uint64_t* bounds, uint64_t n, uint32_t op, bool oriented,
                             uint64_t x, uint64_t xr, uint64_t* out)
     {
         std::lock_guard<std::mutex> lock(mu);
         ensure_ring();
-        struct Flight { uint64_t batch; int slot; uint64_t positions; };
+        // ev: the ring event recorded after the copy of this batch's slot
+        // (-1 while the slot has not been copied yet).
+        struct Flight { uint64_t batch; int slot; uint64_t positions; int ev; };
         std::deque<Flight> q;
         uint64_t in_flight = 0, total = 0, next_slot = 0;
+        int open_begin = -1, open_end = -1;   // ring slots launched, not yet copied
+
+        const auto publish_open = [&]()
+        {
+            publish_counters(d_ring + open_begin, h_ring + open_begin, static_cast<uint64_t>(open_end - open_begin + 1));
+            HIP(hipEventRecord(ring_ev[open_end], stream));
+            for(Flight& f : q)
+                if(f.ev < 0)
+                    f.ev = open_end;
+            open_begin = open_end = -1;
+        };
 
         const auto resolve = [&]()
         {
+            if(q.front().ev < 0)
+                publish_open();
             const Flight f = q.front();
             q.pop_front();
-            HIP(hipEventSynchronize(ring_ev[f.slot]));
+            HIP(hipEventSynchronize(ring_ev[f.ev]));
             const Counters c = h_ring[f.slot];
             in_flight -= f.positions;
             if(c.failed)
@@ -1219,6 +1238,13 @@ struct kache_table
                 sync();
                 grow(d);
             }
+        };
+
+        const auto fits_now = [&](uint64_t pos)
+        {
+            const uint64_t keys_bound = size + in_flight + pos;
+            return q.size() < RING && keys_bound < resize_th && ov_count + in_flight + pos + 1 <= ov_slots &&
+                   static_cast<double>(ov_count + in_flight + pos) < 0.75 * static_cast<double>(ov_slots);
         };
 
         for(uint64_t i = 0; i < n; ++i)
@@ -1239,13 +1265,7 @@ struct kache_table
                     out[i] = c;
                 continue;
             }
-            const auto fits_now = [&]()
-            {
-                const uint64_t keys_bound = size + in_flight + pos;
-                return q.size() < RING && keys_bound < resize_th && ov_count + in_flight + pos + 1 <= ov_slots &&
-                       static_cast<double>(ov_count + in_flight + pos) < 0.75 * static_cast<double>(ov_slots);
-            };
-            while(!q.empty() && !fits_now())
+            while(!q.empty() && !fits_now(pos))
                 resolve();
             if(q.empty())
             {
@@ -1254,10 +1274,14 @@ struct kache_table
             }
             const int slot = static_cast<int>(next_slot++ % RING);
             timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot); });
-            publish_counters(d_ring + slot, h_ring + slot);
-            HIP(hipEventRecord(ring_ev[slot], stream));
-            q.push_back({i, slot, pos});
+            q.push_back({i, slot, pos, -1});
             in_flight += pos;
+            if(open_begin < 0)
+                open_begin = slot;
+            open_end = slot;
+            const uint64_t next_pos = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
+            if(open_end - open_begin + 1 == PUBLISH_GROUP || slot == RING - 1 || i + 1 == n || next_pos > LAUNCH_POS || !fits_now(next_pos))
+                publish_open();
         }
         while(!q.empty())
             resolve();
@@ -1301,6 +1325,7 @@ struct kache_table
     }
 
     static constexpr int RING = 16;         // counter slots of batches in flight
+    static constexpr int PUBLISH_GROUP = 4; // batches whose counters one copy kernel reads back
     mutable Counters* d_ring = nullptr;
     mutable Counters* h_ring = nullptr;
     mutable hipEvent_t ring_ev[RING] = {};
