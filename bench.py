@@ -12,6 +12,7 @@ the insert kernel's roofline position, and the reference CPU path timed on a
 bounded sample of the same reads on this host's cores.
 """
 import argparse
+import collections
 import ctypes as C
 import json
 import os
@@ -286,7 +287,8 @@ def served_probe(a, KA, capi, api, t, u):
     serving workload's mix and shapes drawn from the same genome (count,
     lookup, update +1, upsert +1), run by the serving loop and by plain
     per-request calls, closed loop and at Poisson arrivals of half the plain
-    path's capacity.  After the timed region; its upserts change the table."""
+    path's capacity (two alternating runs of each, latencies pooled).  After
+    the timed region; its upserts change the table."""
     g = capi.DeviceBuffer(api, a.genome)
     api.check(api.lib.kache_amd_synth_genome(g.ptr, a.genome, a.seed), "synth_genome")
     genome = g.download(a.genome)
@@ -298,12 +300,16 @@ def served_probe(a, KA, capi, api, t, u):
             arr.fill(1)
     WAVE = 64 << 20
     serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)     # warm-up
-    cap_s = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)
-    cap_p = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1)
+    ro = [m for m, (k, _) in zip(made, reqs) if k in (capi.REQ_COUNT, capi.REQ_LOOKUP)]
+    cap_s = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20, ro)
+    cap_p = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1, ro)
     rate = 0.5 * cap_p["requests_per_s"]
     arr = np.cumsum(np.random.default_rng(a.seed + 6).exponential(1.0 / rate, len(reqs)))
-    at_s = serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20)
-    at_p = serve_run(capi, t, u, reqs, made, arr, WAVE, 1)
+    runs_s, runs_p = [], []
+    for _ in range(2):      # alternating, on the same arrivals; latencies pooled
+        runs_s.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
+        runs_p.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro))
+    at_s, at_p = pooled(runs_s), pooled(runs_p)
     print(f"bench: served {cap_s['requests_per_s']:.0f} req/s vs plain {cap_p['requests_per_s']:.0f} req/s; at {rate:.0f} req/s "
           f"served p50 {at_s['p50_ms']:.2f} p99 {at_s['p99_ms']:.2f} ms, plain p50 {at_p['p50_ms']:.2f} p99 {at_p['p99_ms']:.2f} ms",
           file=sys.stderr, flush=True)
@@ -358,10 +364,16 @@ def _pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def serve_run(capi, table, user, reqs, made, arrivals, wave_bytes, max_wave):
+def serve_run(capi, table, user, reqs, made, arrivals, wave_bytes, max_wave, warm=None):
     """One server over the table, fed `reqs` (as `made` requests) at
-    `arrivals`; summary dict."""
+    `arrivals`; summary dict.  `warm` (made read-only requests) run first on
+    the same server, closed loop and untimed: a new server's first requests
+    pay one-time costs (its threads' first calls) that are not its steady
+    state.  Wave figures count the timed requests' waves only."""
     with capi.Server(table, user, wave_bytes, max_wave) as s:
+        if warm:
+            s.replay(warm, np.zeros(len(warm)))
+        st0 = s.stats()
         res = s.replay(made, arrivals)
         st = s.stats()
     bad = [r for r in res if r["status"] != 0]
@@ -370,9 +382,27 @@ def serve_run(capi, table, user, reqs, made, arrivals, wave_bytes, max_wave):
     lat = [r["done_s"] - r["submit_s"] for r in res]
     span = max(r["done_s"] for r in res) - min(r["submit_s"] for r in res)
     pos = sum(b.positions() for _, b in reqs)
+    sizes = collections.Counter(r["wave"] for r in res)
     return {"kmers_per_s": pos / span, "requests_per_s": len(reqs) / span, "p50_ms": _pct(lat, 50) * 1e3,
-            "p99_ms": _pct(lat, 99) * 1e3, "mean_ms": float(np.mean(lat)) * 1e3, "waves": st["waves"],
-            "max_wave_requests": st["max_wave_requests"], "seconds": span}
+            "p99_ms": _pct(lat, 99) * 1e3, "mean_ms": float(np.mean(lat)) * 1e3, "waves": st["waves"] - st0["waves"],
+            "direct_waves": st["direct_waves"] - st0["direct_waves"], "max_wave_requests": max(sizes.values()),
+            "seconds": span, "_lat": lat, "_pos": pos}
+
+
+def pooled(runs):
+    """Runs of the same requests and arrivals as one sample: latencies
+    pooled, rates over the summed spans."""
+    lat = [x for r in runs for x in r["_lat"]]
+    span = sum(r["seconds"] for r in runs)
+    return {"kmers_per_s": sum(r["_pos"] for r in runs) / span, "requests_per_s": len(lat) / span,
+            "p50_ms": _pct(lat, 50) * 1e3, "p99_ms": _pct(lat, 99) * 1e3, "mean_ms": float(np.mean(lat)) * 1e3,
+            "waves": sum(r["waves"] for r in runs), "direct_waves": sum(r["direct_waves"] for r in runs),
+            "max_wave_requests": max(r["max_wave_requests"] for r in runs), "seconds": span, "runs": len(runs),
+            "p99_ms_per_run": [r["p99_ms"] for r in runs]}
+
+
+def public(summary):
+    return {k: v for k, v in summary.items() if not k.startswith("_")}
 
 
 def cpu_serve(capi, reqs, genome, region, threads):
@@ -498,8 +528,9 @@ def serve_main(a, KA, capi, api):
             arr.fill(1)         # fault the output pages in before any timing
     serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1 << 20)       # warm-up: the first upserts double the table
     serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1)
-    cap_served = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)
-    cap_plain = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1)
+    ro = [m for m, (k, _) in zip(wmade, warm) if k in (capi.REQ_COUNT, capi.REQ_LOOKUP)]
+    cap_served = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20, ro)
+    cap_plain = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1, ro)
     print(f"serve: capacity served {cap_served['requests_per_s']:.0f} req/s, plain {cap_plain['requests_per_s']:.0f} req/s",
           file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed + 3)
@@ -507,10 +538,16 @@ def serve_main(a, KA, capi, api):
     for frac in (0.25, 0.5, 0.75, 0.9, 1.5, 2.5):
         rate = frac * cap_plain["requests_per_s"]
         arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
-        row = {"rate_req_per_s": rate, "of_plain_capacity": frac,
-               "served": serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20)}
+        row = {"rate_req_per_s": rate, "of_plain_capacity": frac}
         if frac <= 0.9:
-            row["plain"] = serve_run(capi, t, u, reqs, made, arr, WAVE, 1)
+            # Served and plain alternate, twice each, on the same arrivals.
+            runs = {"served": [], "plain": []}
+            for _ in range(2):
+                runs["served"].append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
+                runs["plain"].append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro))
+            row["served"], row["plain"] = pooled(runs["served"]), pooled(runs["plain"])
+        else:
+            row["served"] = public(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
         sweep.append(row)
         print(f"serve: rate {rate:.0f}/s served p50 {row['served']['p50_ms']:.2f} p99 {row['served']['p99_ms']:.2f} ms"
               + (f", plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms" if "plain" in row else ""),
@@ -533,7 +570,7 @@ def serve_main(a, KA, capi, api):
                                "(log-uniform), lengths lognormal median 300 (max 50k), 1% substitutions; Poisson arrivals",
                    "requests": len(reqs), "kmer_positions": pos, "wave_bytes": WAVE, "genome_bp": n},
         "p50_ms": cap_served["p50_ms"], "p99_ms": cap_served["p99_ms"],
-        "capacity": {"served": cap_served, "plain_per_request": cap_plain},
+        "capacity": {"served": public(cap_served), "plain_per_request": public(cap_plain)},
         "sweep": sweep,
         "table": {"keys": t.size(), "buckets": info["buckets"], "resizes": info["resizes"], "device_bytes": info["device_bytes"]},
         "cpu_baseline": cpu,

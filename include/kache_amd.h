@@ -263,6 +263,7 @@ typedef struct kache_amd_server_stats
     uint64_t requests;
     uint64_t max_wave_requests;
     uint64_t max_wave_bytes;
+    uint64_t direct_waves;      /* lone requests run at an idle server by the plain calls, unstaged */
 } kache_amd_server_stats;
 
 typedef struct kache_amd_server kache_amd_server;

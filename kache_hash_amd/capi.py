@@ -61,7 +61,7 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("waves", C.c_uint64), ("requests", C.c_uint64), ("max_wave_requests", C.c_uint64),
-                ("max_wave_bytes", C.c_uint64)]
+                ("max_wave_bytes", C.c_uint64), ("direct_waves", C.c_uint64)]
 
 
 KEYS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_size_t)

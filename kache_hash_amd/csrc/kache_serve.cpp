@@ -6,8 +6,9 @@
 // kernel), its updates, then its upserts, packed into one device text and
 // inserted by one pipelined batch call per run of equal op and operand (each
 // request one batch, so its count of new keys is its own).  Requests arriving
-// while a wave runs join the next one.  Each request sees exactly what a
-// serial run in served order (wave, phase, ticket) gives it.
+// while a wave runs join the next one; a lone request reaching an idle server
+// runs at once by the plain calls, unstaged.  Each request sees exactly what
+// a serial run in served order (wave, phase, ticket) gives it.
 //
 // It uses only the C ABI and the device extensions, like any other client.
 
@@ -66,6 +67,28 @@ uint64_t positions_of(const kache_amd_request& r)
     return p;
 }
 
+// How long a server thread polls for work before it sleeps on its condition
+// variable: a sleeping thread takes tens of microseconds to run again after a
+// notify, which a request arriving at an idle server would pay at every
+// hand-off between threads.
+constexpr double POLL_S = 2e-3;
+
+// cv.wait(lock, pred), polling pred for up to POLL_S first (the lock is
+// released between polls).
+template <class Pred>
+void poll_wait(std::unique_lock<std::mutex>& lock, std::condition_variable& cv, Pred pred)
+{
+    const double until = now_s() + POLL_S;
+    while(!pred() && now_s() < until)
+    {
+        lock.unlock();
+        for(int i = 0; i < 32; ++i)
+            __builtin_ia32_pause();
+        lock.lock();
+    }
+    cv.wait(lock, pred);
+}
+
 uint32_t phase_of(uint32_t kind)
 {
     return kind == KACHE_AMD_REQ_UPDATE ? 1 : kind == KACHE_AMD_REQ_INSERT ? 2 : 0;
@@ -86,6 +109,7 @@ struct Staged
     std::vector<uint64_t> update_at, update_bytes;             // per update request
     std::vector<uint64_t> insert_bounds;                       // per insert request, within its region
     uint64_t bytes = 0;                                        // staged text bytes
+    bool direct = false;                                       // not staged: run by the plain calls
 };
 
 struct kache_amd_server
@@ -119,19 +143,41 @@ struct kache_amd_server
     bool slot_busy[SLOTS] = {};
     std::deque<std::unique_ptr<Staged>> staged;
     bool stager_done = false;
+    bool exec_busy = false;     // the executor is running a wave
 
     kache_amd_server_stats stats{};
 
     kache_amd_server(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_req, uint32_t width)
         : t(t), user(user), wave_bytes(wave_bytes), max_wave_requests(max_req), width(width)
     {
+    }
+
+    // Allocates the pipeline's two slots at their wave size up front, so
+    // that no wave waits for an allocation (a larger lone request still
+    // grows its slot), then starts the threads.
+    int start()
+    {
         if(max_wave_requests == 1)
-            worker = std::thread([this]{ loop(); });
-        else
         {
-            stager = std::thread([this]{ stage_loop(); });
-            worker = std::thread([this]{ exec_loop(); });
+            worker = std::thread([this]{ loop(); });
+            return KACHE_OK;
         }
+        const uint64_t want = wave_bytes + 4096 + KACHE_AMD_TEXT_PAD;
+        for(int i = 0; i < SLOTS; ++i)
+        {
+            void* hp = nullptr;
+            int st = kache_amd_host_alloc(want, &hp);
+            if(st != KACHE_OK)
+                return st;
+            h_slot[i] = static_cast<char*>(hp);
+            st = kache_amd_device_alloc(want, &d_slot[i]);
+            if(st != KACHE_OK)
+                return st;
+            slot_bytes[i] = want;
+        }
+        stager = std::thread([this]{ stage_loop(); });
+        worker = std::thread([this]{ exec_loop(); });
+        return KACHE_OK;
     }
 
     ~kache_amd_server()
@@ -144,7 +190,8 @@ struct kache_amd_server
         cv_slot.notify_all();
         if(stager.joinable())
             stager.join();
-        worker.join();
+        if(worker.joinable())
+            worker.join();
         if(d_text)
             kache_amd_device_free(d_text);
         if(h_text)
@@ -188,7 +235,7 @@ struct kache_amd_server
             uint64_t wave_id = 0;
             {
                 std::unique_lock<std::mutex> lock(mu);
-                cv_in.wait(lock, [&]{ return stopping || !pending.empty(); });
+                poll_wait(lock, cv_in, [&]{ return stopping || !pending.empty(); });
                 if(pending.empty())
                     return;
                 wave = take_wave(wave_id);
@@ -499,16 +546,28 @@ struct kache_amd_server
             auto w = std::make_unique<Staged>();
             {
                 std::unique_lock<std::mutex> lock(mu);
-                cv_in.wait(lock, [&]{ return stopping || !pending.empty(); });
+                poll_wait(lock, cv_in, [&]{ return stopping || !pending.empty(); });
                 if(pending.empty())
                 {
                     stager_done = true;
                     cv_staged.notify_all();
                     return;
                 }
+                // A lone request reaching an idle server is not staged: the
+                // executor runs it at once by the plain calls (whose own
+                // pipeline copies it), one copy and one hand-off fewer.
+                if(pending.size() == 1 && staged.empty() && !exec_busy)
+                {
+                    w->direct = true;
+                    w->reqs = take_wave(w->id);
+                    stats.direct_waves++;
+                    staged.push_back(std::move(w));
+                    cv_staged.notify_all();
+                    continue;
+                }
                 // A wave is admitted when a slot is free: requests arriving
                 // meanwhile wait, and join it.
-                cv_slot.wait(lock, [&]{ return !slot_busy[0] || !slot_busy[1]; });
+                poll_wait(lock, cv_slot, [&]{ return !slot_busy[0] || !slot_busy[1]; });
                 w->slot = slot_busy[0] ? 1 : 0;
                 slot_busy[w->slot] = true;
                 w->reqs = take_wave(w->id);
@@ -548,23 +607,36 @@ struct kache_amd_server
             std::unique_ptr<Staged> w;
             {
                 std::unique_lock<std::mutex> lock(mu);
-                cv_staged.wait(lock, [&]{ return !staged.empty() || stager_done; });
+                poll_wait(lock, cv_staged, [&]{ return !staged.empty() || stager_done; });
                 if(staged.empty())
                     return;
                 w = std::move(staged.front());
                 staged.pop_front();
+                exec_busy = true;
             }
             try
             {
-                execute(*w);
+                if(w->direct)
+                {
+                    const double t0 = now_s();
+                    for(auto& q : w->reqs)
+                        q->t_start = t0, q->wave = w->id, q->phase = phase_of(q->r.kind);
+                    run(w->reqs);
+                }
+                else
+                    execute(*w);
             }
             catch(...)
             {
                 fail(w->reqs);
             }
             std::lock_guard<std::mutex> lock(mu);
-            slot_busy[w->slot] = false;
-            cv_slot.notify_all();
+            exec_busy = false;
+            if(!w->direct)
+            {
+                slot_busy[w->slot] = false;
+                cv_slot.notify_all();
+            }
         }
     }
 
@@ -646,7 +718,14 @@ int kache_amd_server_new(kache_table* t, uint32_t user, uint64_t wave_bytes, uin
         return st;
     try
     {
-        *out = new kache_amd_server(t, user, wave_bytes, max_wave_requests, static_cast<uint32_t>(info.value_bytes));
+        auto* s = new kache_amd_server(t, user, wave_bytes, max_wave_requests, static_cast<uint32_t>(info.value_bytes));
+        const int started = s->start();
+        if(started != KACHE_OK)
+        {
+            delete s;
+            return started;
+        }
+        *out = s;
         return KACHE_OK;
     }
     catch(...)

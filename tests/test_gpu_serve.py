@@ -4,6 +4,7 @@ the requests alone, one after another, in served order (wave, phase,
 ticket) -- checked against the oracle, and the final table (keys, values,
 capacity) against a second device table fed the same calls serially."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -111,8 +112,9 @@ def same(a, b):
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
 
 
-def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, prefill=None):
-    """Serves `reqs`, then checks every output against the oracle and a
+def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, prefill=None, gaps=None):
+    """Serves `reqs` (one submitter: request i after a pause of gaps[i]
+    seconds, if given), then checks every output against the oracle and a
     serial device table run in served order.  Returns the served results."""
     t = capi.Table(api, width, expected_keys=expected)
     u = t.register()
@@ -128,6 +130,8 @@ def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, pr
         tickets = [None] * len(reqs)
         if threads == 1:
             for i, r in enumerate(made):
+                if gaps is not None and gaps[i] > 0:
+                    time.sleep(gaps[i])
                 tickets[i] = s.submit(r)
         else:
             def submitter(w):
@@ -186,6 +190,18 @@ def test_serve_one_request_per_wave_is_plain_calls(api):
     res, stats = serve(api, 8, reqs, 64 << 20, 1)
     assert stats["waves"] == len(reqs)
     assert [r["ticket"] for r in sorted(res, key=lambda r: r["wave"])] == sorted(r["ticket"] for r in res)
+
+
+def test_serve_spaced_arrivals_run_lone_requests_unstaged(api):
+    """Requests 3 ms apart reach an idle server alone and run unstaged by
+    the plain calls; a burst after them is merged into staged waves; both
+    kinds of wave interleave in served order."""
+    g = G.genome(150_000, 57)
+    reqs = make_requests(7, g, 120, 4)
+    gaps = [0.003 if i < 40 or i % 10 == 0 else 0.0 for i in range(len(reqs))]
+    print("serve u32, 120 requests, spaced then bursts", flush=True)
+    res, stats = serve(api, 4, reqs, 64 << 20, 1 << 20, prefill=[g[:40_000]], gaps=gaps)
+    assert 0 < stats["direct_waves"] < stats["waves"] < len(reqs)
 
 
 def test_serve_set_table_rejects_updates_per_request(api):
