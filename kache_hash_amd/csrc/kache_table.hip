@@ -1192,8 +1192,8 @@ struct kache_table
     // schedule, applied batch by batch as their counts arrive, is exact.
     // Each batch counts into its own ring slot; the slots of up to
     // PUBLISH_GROUP consecutive batches are copied to host memory by one
-    // kernel, earlier when the next batch could not be launched before a
-    // count is read.
+    // kernel, earlier when one of the next two batches could not be launched
+    // before a count is read.
     uint64_t insert_batches(uint32_t user, const uint8_t* text, const uint64_t* bounds, uint64_t n, uint32_t op, bool oriented,
                             uint64_t x, uint64_t xr, uint64_t* out)
     {
@@ -1240,12 +1240,19 @@ struct kache_table
             }
         };
 
-        const auto fits_now = [&](uint64_t pos)
+        // Whether a launch of `pos` positions could go now, with `ahead` more
+        // positions in `q_ahead` more launches in flight than now: no
+        // doubling can fall due among them, and the overflow table has a free
+        // slot for every position in flight.  (Its load is kept below 0.75 by
+        // ov_room whenever the pipeline drains; the positions in flight bound
+        // the keys it can gain, of which few spill that far.)
+        const auto fits = [&](uint64_t ahead, uint64_t pos, size_t q_ahead)
         {
-            const uint64_t keys_bound = size + in_flight + pos;
-            return q.size() < RING && keys_bound < resize_th && ov_count + in_flight + pos + 1 <= ov_slots &&
-                   static_cast<double>(ov_count + in_flight + pos) < 0.75 * static_cast<double>(ov_slots);
+            const uint64_t f = in_flight + ahead;
+            return q.size() + q_ahead < RING && size + f + pos < resize_th && ov_count + f + pos + 1 <= ov_slots;
         };
+        const auto fits_now = [&](uint64_t pos) { return fits(0, pos, 0); };
+        const auto batch_pos = [&](uint64_t i) { return i < n ? bounds[i + 1] - bounds[i] : 0; };
 
         for(uint64_t i = 0; i < n; ++i)
         {
@@ -1279,8 +1286,11 @@ struct kache_table
             if(open_begin < 0)
                 open_begin = slot;
             open_end = slot;
-            const uint64_t next_pos = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
-            if(open_end - open_begin + 1 == PUBLISH_GROUP || slot == RING - 1 || i + 1 == n || next_pos > LAUNCH_POS || !fits_now(next_pos))
+            // The counts are copied now if the host may have to read them
+            // before launching batch i + 2: it then still has batch i + 1 queued.
+            const uint64_t p1 = batch_pos(i + 1), p2 = batch_pos(i + 2);
+            const bool needed_soon = i + 1 == n || p1 > LAUNCH_POS || p2 > LAUNCH_POS || !fits_now(p1) || !fits(p1, p2, 1);
+            if(open_end - open_begin + 1 == PUBLISH_GROUP || slot == RING - 1 || needed_soon)
                 publish_open();
         }
         while(!q.empty())
