@@ -52,9 +52,34 @@ struct DeferRing
     uint32_t cf[DEFER_RING];   // checksum | forward << 8
 };
 
-template <bool MAP, uint32_t OP, bool ORIENT>
-__global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt)
+// The counters of the previous launch of a pipeline, handed to the host by
+// the next launch: `src` (complete, since launches on a stream run in order)
+// is copied to host memory and cleared, then `flag` is set to `seq`.
+struct Handoff
 {
+    uint64_t* src;
+    uint64_t* dst;
+    uint64_t* flag;
+    uint64_t seq;
+};
+
+__device__ __forceinline__ void hand_off(const Handoff& h)
+{
+    if(h.src && blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        for(uint32_t i = 0; i < sizeof(Counters) / 8; ++i)
+        {
+            __hip_atomic_store(h.dst + i, h.src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            h.src[i] = 0;
+        }
+        __hip_atomic_store(h.flag, h.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+template <bool MAP, uint32_t OP, bool ORIENT>
+__global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
+{
+    hand_off(prev);
     __shared__ WaveLds wave_lds[TILE / 64];
     __shared__ uint64_t red[TILE / 64];
     __shared__ DeferRing rings[TILE / 64];

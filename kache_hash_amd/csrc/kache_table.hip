@@ -409,6 +409,8 @@ struct kache_table
             (void)hipFree(d_ring), d_ring = nullptr;
         if(h_ring)
             (void)hipHostFree(h_ring), h_ring = nullptr;
+        if(h_seq)
+            (void)hipHostFree(h_seq), h_seq = nullptr;
         for(auto& e : ring_ev)
             if(e)
                 (void)hipEventDestroy(e), e = nullptr;
@@ -818,22 +820,29 @@ struct kache_table
         }
     }
 
-    // The arguments of one insert launch, in k_insert's order.
+    // The arguments of one insert launch, in k_insert's order.  bind() once
+    // the object is in place.
     struct InsertArgs
     {
         TableView T;
         const uint8_t* text;
         uint64_t text_len, b, e, x, xr;
         Counters* cnt;
-        void* ptrs[8];
-        void bind() { ptrs[0] = &T, ptrs[1] = &text, ptrs[2] = &text_len, ptrs[3] = &b, ptrs[4] = &e, ptrs[5] = &x, ptrs[6] = &xr, ptrs[7] = &cnt; }
+        Handoff prev;
+        void* ptrs[9];
+        void bind()
+        {
+            void* p[9] = {&T, &text, &text_len, &b, &e, &x, &xr, &cnt, &prev};
+            std::copy(p, p + 9, ptrs);
+        }
     };
 
-    InsertArgs insert_args(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr, Counters* cnt, bool oriented) const
+    void insert_args(InsertArgs& a, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr, Counters* cnt,
+                     bool oriented, const Handoff& prev = {}) const
     {
-        InsertArgs a{view(), text, text_len, b, e, map ? x : 0, (map && oriented) ? xr : 0, cnt ? cnt : d_cnt, {}};
+        a.T = view(), a.text = text, a.text_len = text_len, a.b = b, a.e = e;
+        a.x = map ? x : 0, a.xr = (map && oriented) ? xr : 0, a.cnt = cnt ? cnt : d_cnt, a.prev = prev;
         a.bind();
-        return a;
     }
 
     uint32_t insert_grid(const void* kern, uint64_t b, uint64_t e) const
@@ -842,10 +851,11 @@ struct kache_table
     }
 
     void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr,
-                         Counters* cnt = nullptr) const
+                         Counters* cnt = nullptr, const Handoff& prev = {}) const
     {
         const void* kern = insert_kernel(op, oriented);
-        InsertArgs a = insert_args(text, text_len, b, e, x, xr, cnt, oriented);
+        InsertArgs a;
+        insert_args(a, text, text_len, b, e, x, xr, cnt, oriented, prev);
         HIP(hipLaunchKernel(kern, dim3(insert_grid(kern, b, e)), dim3(TILE), a.ptrs, 0, stream));
     }
 
@@ -894,7 +904,8 @@ struct kache_table
     void graph_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr) const
     {
         const void* kern = insert_kernel(op, oriented);
-        InsertArgs a = insert_args(text, text_len, b, e, x, xr, nullptr, oriented);
+        InsertArgs a;
+        insert_args(a, text, text_len, b, e, x, xr, nullptr, oriented);
         const uint32_t grid = insert_grid(kern, b, e);
         auto& g = graphs[kern];
         if(!g.exec)
@@ -1190,39 +1201,49 @@ struct kache_table
     // in flight cannot reach the growth threshold nor fill the overflow
     // table: then no doubling can fall due among them, and the growth
     // schedule, applied batch by batch as their counts arrive, is exact.
-    // Each batch counts into its own ring slot; the slots of up to
-    // PUBLISH_GROUP consecutive batches are copied to host memory by one
-    // kernel, earlier when one of the next two batches could not be launched
-    // before a count is read.
+    // Each batch counts into its own ring slot.  The next batch's kernel
+    // hands a batch's counters to the host (copy, clear, then a sequence
+    // flag the host polls); a copy kernel does it instead when the host
+    // needs the count before it can launch the next batch.
     uint64_t insert_batches(uint32_t user, const uint8_t* text, const uint64_t* bounds, uint64_t n, uint32_t op, bool oriented,
                             uint64_t x, uint64_t xr, uint64_t* out)
     {
         std::lock_guard<std::mutex> lock(mu);
         ensure_ring();
-        // ev: the ring event recorded after the copy of this batch's slot
-        // (-1 while the slot has not been copied yet).
-        struct Flight { uint64_t batch; int slot; uint64_t positions; int ev; };
+        // How a batch's counters reach the host: ev >= 0, the ring event
+        // recorded after their copy kernel; seq > 0, the flag value the next
+        // batch's kernel sets; neither yet while both are -1 / 0.
+        struct Flight { uint64_t batch; int slot; uint64_t positions; int ev; uint64_t seq; };
         std::deque<Flight> q;
         uint64_t in_flight = 0, total = 0, next_slot = 0;
-        int open_begin = -1, open_end = -1;   // ring slots launched, not yet copied
 
-        const auto publish_open = [&]()
+        const auto publish = [&](Flight& f)
         {
-            publish_counters(d_ring + open_begin, h_ring + open_begin, static_cast<uint64_t>(open_end - open_begin + 1));
-            HIP(hipEventRecord(ring_ev[open_end], stream));
-            for(Flight& f : q)
-                if(f.ev < 0)
-                    f.ev = open_end;
-            open_begin = open_end = -1;
+            publish_counters(d_ring + f.slot, h_ring + f.slot);
+            HIP(hipEventRecord(ring_ev[f.slot], stream));
+            f.ev = f.slot;
         };
 
         const auto resolve = [&]()
         {
-            if(q.front().ev < 0)
-                publish_open();
+            if(q.front().ev < 0 && q.front().seq == 0)
+                publish(q.front());
             const Flight f = q.front();
             q.pop_front();
-            HIP(hipEventSynchronize(ring_ev[f.ev]));
+            if(f.seq)
+            {
+                // The flag is set once the next batch's kernel starts; if the
+                // stream has drained without it, the hand-off was lost.
+                for(uint64_t spins = 1; __atomic_load_n(h_seq + f.slot, __ATOMIC_ACQUIRE) != f.seq; ++spins)
+                {
+                    __builtin_ia32_pause();
+                    if(spins % 4096 == 0 && hipStreamQuery(stream) != hipErrorNotReady &&
+                       __atomic_load_n(h_seq + f.slot, __ATOMIC_ACQUIRE) != f.seq)
+                        throw hip_failure("a batch's counters were not handed to the host");
+                }
+            }
+            else
+                HIP(hipEventSynchronize(ring_ev[f.ev]));
             const Counters c = h_ring[f.slot];
             in_flight -= f.positions;
             if(c.failed)
@@ -1240,19 +1261,15 @@ struct kache_table
             }
         };
 
-        // Whether a launch of `pos` positions could go now, with `ahead` more
-        // positions in `q_ahead` more launches in flight than now: no
-        // doubling can fall due among them, and the overflow table has a free
-        // slot for every position in flight.  (Its load is kept below 0.75 by
-        // ov_room whenever the pipeline drains; the positions in flight bound
-        // the keys it can gain, of which few spill that far.)
-        const auto fits = [&](uint64_t ahead, uint64_t pos, size_t q_ahead)
+        // Whether a launch of `pos` positions could go now: no doubling can
+        // fall due among the launches in flight, and the overflow table has a
+        // free slot for every position in flight.  (Its load is kept below
+        // 0.75 by ov_room whenever the pipeline drains; the positions in
+        // flight bound the keys it can gain, of which few spill that far.)
+        const auto fits_now = [&](uint64_t pos)
         {
-            const uint64_t f = in_flight + ahead;
-            return q.size() + q_ahead < RING && size + f + pos < resize_th && ov_count + f + pos + 1 <= ov_slots;
+            return q.size() < RING && size + in_flight + pos < resize_th && ov_count + in_flight + pos + 1 <= ov_slots;
         };
-        const auto fits_now = [&](uint64_t pos) { return fits(0, pos, 0); };
-        const auto batch_pos = [&](uint64_t i) { return i < n ? bounds[i + 1] - bounds[i] : 0; };
 
         for(uint64_t i = 0; i < n; ++i)
         {
@@ -1280,18 +1297,20 @@ struct kache_table
                 ov_room(pos);
             }
             const int slot = static_cast<int>(next_slot++ % RING);
-            timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot); });
-            q.push_back({i, slot, pos, -1});
+            Handoff prev{};
+            if(!q.empty() && q.back().ev < 0 && q.back().seq == 0)
+            {
+                Flight& p = q.back();
+                p.seq = ++ring_seq;
+                prev = {reinterpret_cast<uint64_t*>(d_ring + p.slot), reinterpret_cast<uint64_t*>(h_ring + p.slot), h_seq + p.slot, p.seq};
+            }
+            timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot, prev); });
+            q.push_back({i, slot, pos, -1, 0});
             in_flight += pos;
-            if(open_begin < 0)
-                open_begin = slot;
-            open_end = slot;
-            // The counts are copied now if the host may have to read them
-            // before launching batch i + 2: it then still has batch i + 1 queued.
-            const uint64_t p1 = batch_pos(i + 1), p2 = batch_pos(i + 2);
-            const bool needed_soon = i + 1 == n || p1 > LAUNCH_POS || p2 > LAUNCH_POS || !fits_now(p1) || !fits(p1, p2, 1);
-            if(open_end - open_begin + 1 == PUBLISH_GROUP || slot == RING - 1 || needed_soon)
-                publish_open();
+            // The count is needed before the next launch: copy it now.
+            const uint64_t next = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
+            if(i + 1 == n || next > LAUNCH_POS || !fits_now(next))
+                publish(q.back());
         }
         while(!q.empty())
             resolve();
@@ -1335,9 +1354,10 @@ struct kache_table
     }
 
     static constexpr int RING = 16;         // counter slots of batches in flight
-    static constexpr int PUBLISH_GROUP = 4; // batches whose counters one copy kernel reads back
     mutable Counters* d_ring = nullptr;
     mutable Counters* h_ring = nullptr;
+    mutable uint64_t* h_seq = nullptr;      // per slot: the last hand-off's sequence number
+    mutable uint64_t ring_seq = 0;
     mutable hipEvent_t ring_ev[RING] = {};
 
     void ensure_ring() const
@@ -1346,6 +1366,8 @@ struct kache_table
             return;
         d_ring = dev_alloc<Counters>(RING);
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ring), RING * sizeof(Counters), hipHostMallocDefault));
+        HIP(hipHostMalloc(reinterpret_cast<void**>(&h_seq), RING * sizeof(uint64_t), hipHostMallocDefault));
+        std::memset(h_seq, 0, RING * sizeof(uint64_t));
         for(auto& e : ring_ev)
             HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * sizeof(Counters) / 8);
