@@ -392,6 +392,7 @@ struct kache_table
 
     void release()
     {
+        wait_frees();
         if(stream)
             (void)hipStreamSynchronize(stream);
         free_graphs();
@@ -586,10 +587,36 @@ struct kache_table
 
     // --- Resize: doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681) ---
 
+    // The arrays a doubling replaced are freed by a helper thread, so that
+    // the launches after it need not wait for the unmapping; those frees are
+    // waited for before the next doubling allocates and when the table is
+    // released, so at most one set is pending.
+    std::vector<std::thread> freeing;
+
+    void free_later(std::vector<void*> ptrs)
+    {
+        int dev = 0;
+        HIP(hipGetDevice(&dev));
+        freeing.emplace_back([dev, ptrs]{
+            (void)hipSetDevice(dev);
+            for(void* p : ptrs)
+                if(p)
+                    (void)hipFree(p);
+        });
+    }
+
+    void wait_frees()
+    {
+        for(auto& t : freeing)
+            t.join();
+        freeing.clear();
+    }
+
     void resize()
     {
         if(2 * cap > MAX_CAP)
             throw hip_failure("growing the table past the most buckets supported");
+        wait_frees();
 
         const uint64_t ncap = 2 * cap;
         // :1387-1392: the new overflow table takes the old one's keys' room, at least.
@@ -629,8 +656,8 @@ struct kache_table
         if(c.failed)
             throw hip_failure("resize found no room in the overflow table");
 
-        HIP(hipFree(buckets));
-        HIP(hipFree(ov));
+        free_later({buckets, ov, away.p});
+        away.p = nullptr, away.bytes = 0;
         buckets = nb, cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = c.ov_inserted;
         resizes++;
         resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
