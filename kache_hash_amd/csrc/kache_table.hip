@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <future>
 #include <map>
 #include <mutex>
 #include <new>
@@ -393,6 +394,8 @@ struct kache_table
     void release()
     {
         wait_frees();
+        if(prepared.valid())
+            free_prepared(prepared.get());
         if(stream)
             (void)hipStreamSynchronize(stream);
         free_graphs();
@@ -612,6 +615,64 @@ struct kache_table
         freeing.clear();
     }
 
+    // The arrays of the next doubling, allocated ahead by a helper thread
+    // once one is near: mapping gigabytes takes the host a while, which would
+    // otherwise stall the launches around the doubling.
+    struct Prepared
+    {
+        uint64_t cap = 0, ov_slots = 0, away_bytes = 0;
+        void* buckets = nullptr;
+        void* ov = nullptr;
+        void* away = nullptr;
+    };
+    std::future<Prepared> prepared;
+
+    uint64_t next_ov_slots() const { return std::max<uint64_t>(ov_slots_for(std::max<uint64_t>(OV_MIN, 2 * cap)), ov_slots); }
+
+    void prepare_resize()
+    {
+        if(prepared.valid() || 2 * cap > MAX_CAP)
+            return;
+        wait_frees();
+        Prepared p;
+        p.cap = 2 * cap, p.ov_slots = next_ov_slots(), p.away_bytes = std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
+        const uint64_t bucket_b = p.cap * bucket_bytes(), ov_b = p.ov_slots * slot_words() * 8;
+        int dev = 0;
+        HIP(hipGetDevice(&dev));
+        prepared = std::async(std::launch::async, [p, bucket_b, ov_b, dev]() mutable
+        {
+            (void)hipSetDevice(dev);
+            if(hipMalloc(&p.buckets, bucket_b) != hipSuccess)
+                p.buckets = nullptr;
+            if(hipMalloc(&p.ov, ov_b) != hipSuccess)
+                p.ov = nullptr;
+            if(hipMalloc(&p.away, p.away_bytes) != hipSuccess)
+                p.away = nullptr;
+            return p;
+        });
+    }
+
+    static void free_prepared(const Prepared& p)
+    {
+        for(void* q : {p.buckets, p.ov, p.away})
+            if(q)
+                (void)hipFree(q);
+    }
+
+    // The prepared arrays, if they are those of the doubling about to run.
+    Prepared take_prepared(uint64_t ncap, uint64_t nov_slots)
+    {
+        if(!prepared.valid())
+            return {};
+        Prepared p = prepared.get();
+        if(p.cap != ncap || p.ov_slots != nov_slots || !p.buckets || !p.ov || !p.away)
+        {
+            free_prepared(p);
+            return {};
+        }
+        return p;
+    }
+
     void resize()
     {
         if(2 * cap > MAX_CAP)
@@ -620,24 +681,34 @@ struct kache_table
 
         const uint64_t ncap = 2 * cap;
         // :1387-1392: the new overflow table takes the old one's keys' room, at least.
-        const uint64_t nov_slots = std::max<uint64_t>(ov_slots_for(std::max<uint64_t>(OV_MIN, ncap)), ov_slots);
-        uint8_t* nb = dev_alloc<uint8_t>(ncap * bucket_bytes());   // written whole by the split
+        const uint64_t nov_slots = next_ov_slots();
+        const uint64_t pairs = (cap + 1) / 2;
+        const Prepared pre = take_prepared(ncap, nov_slots);
+        uint8_t* nb = nullptr;      // written whole by the split
         uint64_t* nov = nullptr;
-        try
+        DevBuf away;                // one bit per old slot: the keys away from their primary bucket
+        if(pre.buckets)
         {
-            nov = alloc_ov(nov_slots);
+            nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
+            zero(nov, nov_slots * slot_words() * 8);
+            away.p = pre.away, away.bytes = pre.away_bytes;
         }
-        catch(...)
+        else
         {
-            (void)hipFree(nb);
-            throw;
+            nb = dev_alloc<uint8_t>(ncap * bucket_bytes());
+            try
+            {
+                nov = alloc_ov(nov_slots);
+            }
+            catch(...)
+            {
+                (void)hipFree(nb);
+                throw;
+            }
+            away.reserve(pairs * 8);
         }
 
         const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
-        // One bit per old slot marks the keys away from their primary bucket.
-        const uint64_t pairs = (cap + 1) / 2;
-        DevBuf away;
-        away.reserve(pairs * 8);
         const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
         const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
         timed("split", [&]{
@@ -1063,6 +1134,8 @@ struct kache_table
                 launch_log2 = trials[tune_next % trials.size()].cfg.launch_log2;
             const uint64_t e = std::min(text_len, b + (uint64_t(1) << launch_log2));
             ov_room(e - b);
+            if(size + 2 * (e - b) >= resize_th)
+                prepare_resize();
             const Counters c = insert_launch(op, oriented, text, text_len, b, e, x, xr);
             if(c.failed)
                 throw hip_failure("an insert found no room in the overflow table");
@@ -1336,6 +1409,8 @@ struct kache_table
             in_flight += pos;
             // The count is needed before the next launch: copy it now.
             const uint64_t next = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
+            if(size + in_flight + 2 * next >= resize_th)
+                prepare_resize();
             if(i + 1 == n || next > LAUNCH_POS || !fits_now(next))
                 publish(q.back());
         }
