@@ -1956,6 +1956,10 @@ struct kache_table
     {
         std::lock_guard<std::mutex> lock(mu);
         sync();
+        // Arrays prepared for the old table's next doubling are of no use now.
+        wait_frees();
+        if(prepared.valid())
+            free_prepared(prepared.get());
         uint8_t* nb = dev_alloc<uint8_t>(h.cap * bucket_bytes());
         uint64_t* nov = nullptr;
         try
