@@ -66,7 +66,7 @@ def parse():
     p.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     p.add_argument("--cpu-threads", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--served-requests", type=int, default=200,
+    p.add_argument("--served-requests", type=int, default=600,
                    help="count-map: requests of the served-latency probe on the built table (0: none)")
     p.add_argument("--blocks-per-cu", type=int, default=0)
     p.add_argument("--seed", type=int, default=None)
@@ -305,6 +305,8 @@ def served_probe(a, KA, capi, api, t, u):
     cap_p = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1, ro)
     rate = 0.5 * cap_p["requests_per_s"]
     arr = np.cumsum(np.random.default_rng(a.seed + 6).exponential(1.0 / rate, len(reqs)))
+    serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro)       # untimed open-loop warm-up of each
+    serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro)
     runs_s, runs_p = [], []
     for _ in range(2):      # alternating, on the same arrivals; latencies pooled
         runs_s.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
@@ -535,9 +537,15 @@ def serve_main(a, KA, capi, api):
           file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed + 3)
     sweep = []
-    for frac in (0.25, 0.5, 0.75, 0.9, 1.5, 2.5):
+    fracs = (0.25, 0.5, 0.75, 0.9, 1.5, 2.5)
+    arrivals = [np.cumsum(rng.exponential(1.0 / (f * cap_plain["requests_per_s"]), len(reqs))) for f in fracs]
+    # Untimed, one of each: the first open-loop served run after the
+    # closed-loop ones has shown a one-off stall of about a millisecond early
+    # in the run.
+    serve_run(capi, t, u, reqs, made, arrivals[0], WAVE, 1 << 20, ro)
+    serve_run(capi, t, u, reqs, made, arrivals[0], WAVE, 1, ro)
+    for frac, arr in zip(fracs, arrivals):
         rate = frac * cap_plain["requests_per_s"]
-        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
         row = {"rate_req_per_s": rate, "of_plain_capacity": frac}
         if frac <= 0.9:
             # Served and plain alternate, twice each, on the same arrivals.

@@ -2,8 +2,9 @@
 // Requests (per-sequence counts, per-position lookups, present-only updates,
 // upserts) arrive from any thread; one worker admits them in arrival order
 // into waves, as many as fit the wave's staging budget in HBM, and runs each
-// wave as: its reads merged into one call per kind (one staging copy, one
-// kernel), its updates, then its upserts, packed into one device text and
+// wave as: its reads merged into a few calls per kind (one staging copy;
+// smaller requests in earlier calls, since reads commute), its updates, then
+// its upserts, packed into one device text and
 // inserted by one pipelined batch call per run of equal op and operand (each
 // request one batch, so its count of new keys is its own).  Requests arriving
 // while a wave runs join the next one; a lone request reaching an idle server
@@ -96,6 +97,20 @@ uint32_t phase_of(uint32_t kind)
 
 }
 
+// A wave's reads of one kind (counts or lookups) run in chunks of ascending
+// size, one device call each, whose requests complete together: a chunk ends
+// where the next request is larger than the whole chunk, so that no request
+// waits for one much larger than itself, in few calls.  Reads commute, so
+// their order within the phase changes no result.  A wave taken while more
+// requests wait (the server is saturated, and throughput, not the latency of
+// one wave, decides) runs each kind in one call.
+struct ReadChunk
+{
+    size_t begin = 0, end = 0;          // requests [begin, end) of the kind, in size order
+    uint64_t at = 0;                    // start in the slot's text (16-byte aligned)
+    std::vector<uint64_t> off;          // sequence offsets from `at`, and the end
+};
+
 // A wave staged for the device: its requests by phase, and where their
 // sequences sit in the slot's text (each sequence followed by '\n').
 struct Staged
@@ -104,12 +119,13 @@ struct Staged
     int slot = 0;
     std::vector<std::shared_ptr<Req>> reqs;
     std::vector<Req*> counts, lookups, updates, inserts;
-    uint64_t count_at = 0, lookup_at = 0, insert_at = 0;       // region starts (16-byte aligned)
-    std::vector<uint64_t> count_off, lookup_off;               // sequence offsets within their region
+    std::vector<ReadChunk> count_chunks, lookup_chunks;
+    uint64_t insert_at = 0;                                    // region start (16-byte aligned)
     std::vector<uint64_t> update_at, update_bytes;             // per update request
     std::vector<uint64_t> insert_bounds;                       // per insert request, within its region
     uint64_t bytes = 0;                                        // staged text bytes
     bool direct = false;                                       // not staged: run by the plain calls
+    bool saturated = false;                                    // more requests waited when it was taken
 };
 
 struct kache_amd_server
@@ -453,8 +469,27 @@ struct kache_amd_server
             case KACHE_AMD_REQ_UPDATE: w.updates.push_back(q.get()); break;
             default: w.inserts.push_back(q.get()); break;
             }
-        // Layout: counts, lookups, each update request, inserts; every region 16-byte aligned.
+        // Layout: count chunks, lookup chunks, each update request, inserts;
+        // every region 16-byte aligned.
+        const auto by_size = [](const Req* a, const Req* b) { return a->bytes < b->bytes || (a->bytes == b->bytes && a->ticket < b->ticket); };
+        std::sort(w.counts.begin(), w.counts.end(), by_size);
+        std::sort(w.lookups.begin(), w.lookups.end(), by_size);
         uint64_t o = 0;
+        const auto chunks = [&](const std::vector<Req*>& rs, std::vector<ReadChunk>& out)
+        {
+            for(size_t i = 0; i < rs.size();)
+            {
+                ReadChunk c;
+                c.begin = i, c.at = o;
+                uint64_t b = 0;
+                do
+                    b += rs[i++]->bytes;
+                while(i < rs.size() && (w.saturated || rs[i]->bytes <= b));
+                c.end = i;
+                o = align16(o + b);
+                out.push_back(std::move(c));
+            }
+        };
         const auto region = [&](const std::vector<Req*>& rs)
         {
             uint64_t b = 0;
@@ -464,8 +499,8 @@ struct kache_amd_server
             o = align16(o + b);
             return at;
         };
-        w.count_at = region(w.counts);
-        w.lookup_at = region(w.lookups);
+        chunks(w.counts, w.count_chunks);
+        chunks(w.lookups, w.lookup_chunks);
         for(Req* q : w.updates)
             w.update_at.push_back(o), w.update_bytes.push_back(q->bytes), o = align16(o + q->bytes);
         w.insert_at = region(w.inserts);
@@ -512,21 +547,23 @@ struct kache_amd_server
             filled = at;
             return at;
         };
-        uint64_t at = w.count_at;
-        for(Req* q : w.counts)
-            at = put(q->r, at, &w.count_off);
-        w.count_off.push_back(at);
-        for(auto& x : w.count_off)
-            x -= w.count_at;
-        at = w.lookup_at;
-        for(Req* q : w.lookups)
-            at = put(q->r, at, &w.lookup_off);
-        w.lookup_off.push_back(at);
-        for(auto& x : w.lookup_off)
-            x -= w.lookup_at;
+        const auto fill_chunks = [&](const std::vector<Req*>& rs, std::vector<ReadChunk>& cs)
+        {
+            for(ReadChunk& c : cs)
+            {
+                uint64_t a = c.at;
+                for(size_t k = c.begin; k < c.end; ++k)
+                    a = put(rs[k]->r, a, &c.off);
+                c.off.push_back(a);
+                for(auto& x : c.off)
+                    x -= c.at;
+            }
+        };
+        fill_chunks(w.counts, w.count_chunks);
+        fill_chunks(w.lookups, w.lookup_chunks);
         for(size_t i = 0; i < w.updates.size(); ++i)
             put(w.updates[i]->r, w.update_at[i], nullptr);
-        at = w.insert_at;
+        uint64_t at = w.insert_at;
         w.insert_bounds.push_back(0);
         for(Req* q : w.inserts)
         {
@@ -571,6 +608,7 @@ struct kache_amd_server
                 w->slot = slot_busy[0] ? 1 : 0;
                 slot_busy[w->slot] = true;
                 w->reqs = take_wave(w->id);
+                w->saturated = !pending.empty();
             }
             int st = KACHE_ENOMEM;
             try
@@ -647,37 +685,39 @@ struct kache_amd_server
             q->t_start = t0, q->wave = w.id, q->phase = phase_of(q->r.kind);
         const char* d = static_cast<const char*>(d_slot[w.slot]);
 
-        if(!w.counts.empty())
+        for(const ReadChunk& c : w.count_chunks)
         {
-            const size_t n = w.count_off.size() - 1;
+            const size_t n = c.off.size() - 1;
             std::vector<uint64_t> out(std::max<size_t>(n, 1));
-            const int st = kache_amd_count_seqs_device(t, d + w.count_at, w.count_off.back(), w.count_off.data(), n, out.data());
+            const int st = kache_amd_count_seqs_device(t, d + c.at, c.off.back(), c.off.data(), n, out.data());
+            std::vector<Req*> done(w.counts.begin() + c.begin, w.counts.begin() + c.end);
             size_t o = 0;
-            for(Req* q : w.counts)
+            for(Req* q : done)
             {
                 q->status = st;
                 if(st == KACHE_OK && q->r.n)
                     std::memcpy(q->r.counts, out.data() + o, q->r.n * sizeof(uint64_t));
                 o += q->r.n;
             }
-            finish(w.counts);
+            finish(done);
         }
-        if(!w.lookups.empty())
+        for(const ReadChunk& c : w.lookup_chunks)
         {
+            std::vector<Req*> done(w.lookups.begin() + c.begin, w.lookups.begin() + c.end);
             std::vector<uint64_t> first;
             std::vector<uint8_t*> found, fwd;
             std::vector<void*> values;
             uint64_t o = 0;
-            for(Req* q : w.lookups)
+            for(Req* q : done)
             {
                 first.push_back(o), o += q->r.n;
                 found.push_back(q->r.found), values.push_back(q->r.values), fwd.push_back(q->r.fwd);
             }
-            const int st = kache_amd_lookup_groups_device(t, d + w.lookup_at, w.lookup_off.back(), w.lookup_off.data(), w.lookup_off.size() - 1,
-                                                          first.data(), w.lookups.size(), found.data(), values.data(), fwd.data());
-            for(Req* q : w.lookups)
+            const int st = kache_amd_lookup_groups_device(t, d + c.at, c.off.back(), c.off.data(), c.off.size() - 1, first.data(), done.size(),
+                                                          found.data(), values.data(), fwd.data());
+            for(Req* q : done)
                 q->status = st;
-            finish(w.lookups);
+            finish(done);
         }
         for(size_t i = 0; i < w.updates.size(); ++i)
         {
@@ -685,11 +725,16 @@ struct kache_amd_server
             q->status = kache_amd_update_device(t, d + w.update_at[i], w.update_bytes[i], q->r.op, q->r.x, q->r.counts);
         }
         finish(w.updates);
+        // Upserts in ticket order, one pipelined call per run of equal op and
+        // operand; unless the wave is saturated, a call also ends before a
+        // request larger than the call so far, and its requests complete then.
         for(size_t a = 0; a < w.inserts.size();)
         {
             size_t b = a + 1;
-            while(b < w.inserts.size() && w.inserts[b]->r.op == w.inserts[a]->r.op && w.inserts[b]->r.x == w.inserts[a]->r.x)
-                b++;
+            uint64_t bytes = w.inserts[a]->bytes;
+            while(b < w.inserts.size() && w.inserts[b]->r.op == w.inserts[a]->r.op && w.inserts[b]->r.x == w.inserts[a]->r.x &&
+                  (w.saturated || w.inserts[b]->bytes <= bytes))
+                bytes += w.inserts[b++]->bytes;
             std::vector<uint64_t> out(b - a);
             const int st = kache_amd_insert_batches(t, user, d + w.insert_at, w.insert_bounds.data() + a, b - a, w.inserts[a]->r.op,
                                                     w.inserts[a]->r.x, out.data());
@@ -699,9 +744,9 @@ struct kache_amd_server
                 if(st == KACHE_OK && w.inserts[i]->r.counts)
                     *w.inserts[i]->r.counts = out[i - a];
             }
+            finish(std::vector<Req*>(w.inserts.begin() + a, w.inserts.begin() + b));
             a = b;
         }
-        finish(w.inserts);
     }
 };
 
