@@ -1371,52 +1371,65 @@ struct kache_table
             return q.size() < RING && size + in_flight + pos < resize_th && ov_count + in_flight + pos + 1 <= ov_slots;
         };
 
-        for(uint64_t i = 0; i < n; ++i)
+        try
         {
-            const uint64_t b = bounds[i], e = bounds[i + 1];
-            const uint64_t pos = e - b;
-            if(pos > LAUNCH_POS)
+            for(uint64_t i = 0; i < n; ++i)
             {
-                // A batch too large for one launch goes alone, as a call.
-                while(!q.empty())
+                const uint64_t b = bounds[i], e = bounds[i + 1];
+                const uint64_t pos = e - b;
+                if(pos > LAUNCH_POS)
+                {
+                    // A batch too large for one launch goes alone, as a call.
+                    while(!q.empty())
+                        resolve();
+                    sync();
+                    CallSched s = sched_begin(user);
+                    const uint64_t c = insert_text(s, op, oriented, text + (b & ~uint64_t(15)), e - (b & ~uint64_t(15)), x, xr, b & 15);
+                    sched_end(s, user);
+                    total += c;
+                    if(out)
+                        out[i] = c;
+                    continue;
+                }
+                while(!q.empty() && !fits_now(pos))
                     resolve();
-                sync();
-                CallSched s = sched_begin(user);
-                const uint64_t c = insert_text(s, op, oriented, text + (b & ~uint64_t(15)), e - (b & ~uint64_t(15)), x, xr, b & 15);
-                sched_end(s, user);
-                total += c;
-                if(out)
-                    out[i] = c;
-                continue;
+                if(q.empty())
+                {
+                    sync();
+                    ov_room(pos);
+                }
+                const int slot = static_cast<int>(next_slot++ % RING);
+                Handoff prev{};
+                if(!q.empty() && q.back().ev < 0 && q.back().seq == 0)
+                {
+                    Flight& p = q.back();
+                    p.seq = ++ring_seq;
+                    prev = {reinterpret_cast<uint64_t*>(d_ring + p.slot), reinterpret_cast<uint64_t*>(h_ring + p.slot), h_seq + p.slot, p.seq};
+                }
+                timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot, prev); });
+                q.push_back({i, slot, pos, -1, 0});
+                in_flight += pos;
+                const uint64_t next = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
+                if(size + in_flight + 2 * next >= resize_th)
+                    prepare_resize();
+                // The count is needed before the next launch: copy it now.
+                if(i + 1 == n || next > LAUNCH_POS || !fits_now(next))
+                    publish(q.back());
             }
-            while(!q.empty() && !fits_now(pos))
+            while(!q.empty())
                 resolve();
-            if(q.empty())
-            {
-                sync();
-                ov_room(pos);
-            }
-            const int slot = static_cast<int>(next_slot++ % RING);
-            Handoff prev{};
-            if(!q.empty() && q.back().ev < 0 && q.back().seq == 0)
-            {
-                Flight& p = q.back();
-                p.seq = ++ring_seq;
-                prev = {reinterpret_cast<uint64_t*>(d_ring + p.slot), reinterpret_cast<uint64_t*>(h_ring + p.slot), h_seq + p.slot, p.seq};
-            }
-            timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot, prev); });
-            q.push_back({i, slot, pos, -1, 0});
-            in_flight += pos;
-            // The count is needed before the next launch: copy it now.
-            const uint64_t next = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
-            if(size + in_flight + 2 * next >= resize_th)
-                prepare_resize();
-            if(i + 1 == n || next > LAUNCH_POS || !fits_now(next))
-                publish(q.back());
+            sync();
         }
-        while(!q.empty())
-            resolve();
-        sync();
+        catch(...)
+        {
+            // Counts of launches that were in flight stay in their slots:
+            // clear the ring for the next call (it is otherwise zero whenever
+            // no launch is in flight).
+            (void)hipStreamSynchronize(stream);
+            hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * sizeof(Counters) / 8);
+            (void)hipStreamSynchronize(stream);
+            throw;
+        }
         return total;
     }
 
