@@ -88,7 +88,7 @@ __global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restri
     const uint32_t lane = __lane_id();
     const uint64_t below_me = (uint64_t(1) << lane) - 1;
     uint32_t head = 0, queued = 0;   // the wave's ring: entries [head, head + queued) mod DEFER_RING
-    uint64_t ins = 0, ins_ov = 0, failed = 0;
+    uint64_t ins = 0, ins_ov = 0, failed = 0;     // the wave's counts
 
     // Settles the ring entry of each lane with `active` (all lanes take part).
     auto settle = [&](bool active, uint32_t i)
@@ -98,16 +98,13 @@ __global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restri
         uint64_t* slot;
         bool in_ov;
         const Res r = insert_key_wave<MAP>(T, active, key, h, static_cast<uint8_t>(cf), slot, in_ov);
-        if(!active)
-            return;
-        if(r == RES_FULL)
-            failed++;
-        else
-        {
-            ins += (r == RES_NEW), ins_ov += (r == RES_NEW && in_ov);
+        // The wave's counts (uniform: scalar registers).
+        ins += __popcll(__ballot(active && r == RES_NEW));
+        ins_ov += __popcll(__ballot(active && r == RES_NEW && in_ov));
+        failed += __popcll(__ballot(active && r == RES_FULL));
+        if(active && r != RES_FULL)
             if constexpr(MAP)
                 value_op<OP>(slot + VAL_OFF, (ORIENT && !(cf >> 8)) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
-        }
     };
 
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
@@ -141,9 +138,9 @@ __global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restri
         wave_sync();
         settle(lane < queued, (head + lane) % DEFER_RING);
     }
-    block_add(ins, &cnt->inserted, red);
-    block_add(ins_ov, &cnt->ov_inserted, red);
-    block_add(failed, &cnt->failed, red);
+    block_add(lane == 0 ? ins : 0, &cnt->inserted, red);
+    block_add(lane == 0 ? ins_ov : 0, &cnt->ov_inserted, red);
+    block_add(lane == 0 ? failed : 0, &cnt->failed, red);
 }
 
 // Present-only updates (include/Streaming_Kmer_Hash_Table.hpp:2648-2691).
