@@ -408,6 +408,12 @@ constexpr uint32_t CLAIM_ROUNDS = 256;
 
 __device__ __forceinline__ bool is_key(uint64_t v) { return (v >> 62) == 3; }
 
+// The set bits of m in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l)
 {
     return (uint64_t(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), l))) << 32) |
@@ -509,7 +515,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
         if(need && owner < 0)
         {
             const uint64_t rk = sk ^ RESERVE_FLIP;
-            const uint32_t rank = __popcll(grp & leads & below_me);
+            const uint32_t rank = lanes_below(grp & leads);
             const uint32_t j = z + rank;
             // The first of a group has no slot to check below its own (those
             // below z hold keys whose checksums were read): it claims with its
@@ -622,7 +628,6 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
     uint32_t stage = 0, lo = 0;
     uint64_t bidx = active ? bucket_of(h, T.cap) : 0;
     bool need = active;
-    const uint64_t below_me = (uint64_t(1) << __lane_id()) - 1;
     for(uint32_t round = 0; __ballot(need); ++round)
     {
         if(round == CLAIM_ROUNDS)
@@ -653,7 +658,7 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
         }
         if(need)
         {
-            const uint32_t j = z + __popcll(grp & below_me);
+            const uint32_t j = z + lanes_below(grp);
             if(j < B)
             {
                 uint64_t occ = 0;

@@ -8,12 +8,13 @@
 // Occupancy of the streaming kernels: minimum waves per SIMD the compiler
 // must fit, which caps their registers.  The insert kernel's settle path
 // needs more registers than the lookups: at 8 waves its common path reloads
-// spilled values inside the loop; 6 waves with fewer spills measured faster.
+// spilled values inside the loop; 7 waves (72 registers, a few spilled
+// outside the common path) measured fastest, then 6 (80, none spilled).
 #ifndef KAMD_STREAM_WAVES
 #define KAMD_STREAM_WAVES 8
 #endif
 #ifndef KAMD_INSERT_WAVES
-#define KAMD_INSERT_WAVES 6
+#define KAMD_INSERT_WAVES 7
 #endif
 #define KAMD_STREAM_BOUNDS __launch_bounds__(256, KAMD_STREAM_WAVES)
 #define KAMD_INSERT_BOUNDS __launch_bounds__(256, KAMD_INSERT_WAVES)
@@ -86,7 +87,6 @@ __global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restri
     WaveLds& s = wave_lds[threadIdx.x >> 6];
     DeferRing& q = rings[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
-    const uint64_t below_me = (uint64_t(1) << lane) - 1;
     uint32_t head = 0, queued = 0;   // the wave's ring: entries [head, head + queued) mod DEFER_RING
     uint64_t ins = 0, ins_ov = 0, failed = 0;     // the wave's counts
 
@@ -122,7 +122,7 @@ __global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restri
         const uint64_t m = __ballot(defer);
         if(defer)
         {
-            const uint32_t i = (head + queued + __popcll(m & below_me)) % DEFER_RING;
+            const uint32_t i = (head + queued + lanes_below(m)) % DEFER_RING;
             q.key[i] = d.key, q.h[i] = d.h, q.cf[i] = d.c | (d.fwd << 8);
         }
         queued += __popcll(m);
