@@ -123,6 +123,16 @@ int kache_amd_histogram(const kache_table* t, uint64_t max, uint64_t* out) KACHE
  * checksum and orientation.  All pointers are device pointers. */
 int kache_amd_describe_device(const char* text, uint64_t text_bytes, uint64_t* key, uint64_t* h, uint8_t* c, uint8_t* fwd, uint8_t* valid) KACHE_NOEXCEPT;
 
+/* Device runs of the hash and placement functions, for tests: for each of the
+ * n inputs x[i] writes 10 words at out[10 i]: wang_u64(x), the checksum,
+ * the minimizer hash of x & (2^62 - 1), XXH3_64bits_withSeed(&x, 8, 0) and
+ * with seed 2^63, bucket_of(x, cap), filter_pos(x), pick_bucket(x, cap), and
+ * the secondaries p, q of x as a minimizer hash in a table of cap buckets
+ * (include/Kmer_Batch.hpp:24-67, include/Kmer_Lanes.hpp:247-274,
+ * include/xxHash/xxhash.h:4273-4285, include/Streaming_Kmer_Hash_Table.hpp:1014-1068,
+ * 2111-2115).  x and out are device pointers; 1 <= cap <= 2^32 - 1. */
+int kache_amd_hashes_device(const uint64_t* x, uint64_t n, uint64_t cap, uint64_t* out) KACHE_NOEXCEPT;
+
 /* Kernel timing: with profiling on, HIP events bracket every launch on the
  * table's stream; kache_amd_kernel_time puts the summed milliseconds and the
  * number of launches of the kernel family `name` ("insert", "update",
@@ -132,6 +142,16 @@ int kache_amd_kernel_time(const kache_table* t, const char* name, double* ms, ui
 
 /* Blocks per compute unit of the persistent launches (0: as many as fit). */
 int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
+
+/* HBM budget of a table (0, the default: only the device's free memory limits
+ * it): the most bytes its bucket store and overflow table may hold, the new
+ * arrays of a doubling or overflow growth included while both exist.  Growth
+ * that would pass the budget or the device's free memory fails before it
+ * allocates, with KACHE_ENOMEM: the call's launches so far stay applied (as
+ * a call that fails part-way in the reference), the table keeps its capacity
+ * and every result stays exact, and the next flush of the growth schedule
+ * tries the doubling again. */
+int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
 
 /* Launch shape of the streaming inserts of kache_amd_insert_device and
  * kache_amd_insert_oriented_device: blocks per compute unit (0: as many as

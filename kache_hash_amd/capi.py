@@ -128,9 +128,11 @@ AMD_SYMBOLS = {
     "kache_amd_autotune": (C.c_int, [_P, C.c_uint32]),
     "kache_amd_tuning_trials": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "kache_amd_describe_device": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P, _P]),
+    "kache_amd_hashes_device": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P]),
     "kache_amd_profile": (C.c_int, [_P, C.c_int]),
     "kache_amd_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kache_amd_set_blocks_per_cu": (C.c_int, [_P, C.c_uint32]),
+    "kache_amd_set_hbm_budget": (C.c_int, [_P, C.c_uint64]),
     "kache_amd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
     "kache_amd_last_error": (C.c_char_p, []),
     "kache_amd_server_new": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(_P)]),
@@ -460,6 +462,10 @@ class Table:
         self.api.check(self.api.lib.kache_amd_tuning_trials(self.h, out, n.value, C.byref(n)), "kache_amd_tuning_trials")
         return [{"blocks_per_cu": t.cfg.blocks_per_cu, "launch_log2": t.cfg.launch_log2, "graph": t.cfg.graph,
                  "launches": t.launches, "median_ns_per_position": t.median_ns_per_position} for t in out[:n.value]]
+
+    def set_hbm_budget(self, nbytes):
+        """Most bytes of HBM the table may hold (0: no limit but the device's)."""
+        self.api.check(self.api.lib.kache_amd_set_hbm_budget(self.h, nbytes), "kache_amd_set_hbm_budget")
 
     def set_blocks_per_cu(self, blocks):
         self.api.check(self.api.lib.kache_amd_set_blocks_per_cu(self.h, blocks), "kache_amd_set_blocks_per_cu")

@@ -707,8 +707,10 @@ __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ t
     WaveLds& s = wave_lds[threadIdx.x >> 6];
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t tile, const Desc& d)
     {
+        // Tiles start at pos_begin rounded down to 16: positions below
+        // pos_begin are described but have no output slot.
         const uint64_t p = (pos_begin & ~uint64_t(15)) + tile * TILE + threadIdx.x;
-        if(p < pos_end)
+        if(p >= pos_begin && p < pos_end)
         {
             const uint64_t i = p - pos_begin;
             valid[i] = static_cast<uint8_t>(d.valid);
@@ -718,6 +720,34 @@ __global__ __launch_bounds__(TILE) void k_describe(const uint8_t* __restrict__ t
             fwd[i] = d.valid ? static_cast<uint8_t>(d.fwd) : 0;
         }
     });
+}
+
+// The hash and placement functions of the table, evaluated on the device for
+// tests: per input x, HASH_COLUMNS words: wang_u64(x), checksum(x),
+// minimizer_hash(x & (2^62 - 1)), xxh3_8(x, 0), xxh3_8(x, 2^63),
+// bucket_of(x, cap), filter_pos(x), pick_bucket(x, cap) and the secondaries
+// p, q of x as a minimizer hash in a table of cap buckets.
+constexpr uint32_t HASH_COLUMNS = 10;
+
+__global__ __launch_bounds__(TILE) void k_hashes(const uint64_t* __restrict__ x, uint64_t n, uint64_t cap, uint64_t* __restrict__ out)
+{
+    for(uint64_t i = blockIdx.x * uint64_t(TILE) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * TILE)
+    {
+        const uint64_t v = x[i];
+        uint64_t* o = out + i * HASH_COLUMNS;
+        uint64_t p, q;
+        secondaries(v, cap, p, q);
+        o[0] = wang_u64(v);
+        o[1] = checksum(v);
+        o[2] = minimizer_hash(v & KMER_MASK);
+        o[3] = xxh3_8(v, 0);
+        o[4] = xxh3_8(v, uint64_t(1) << 63);
+        o[5] = bucket_of(v, cap);
+        o[6] = filter_pos(v);
+        o[7] = pick_bucket(v, cap);
+        o[8] = p;
+        o[9] = q;
+    }
 }
 
 }

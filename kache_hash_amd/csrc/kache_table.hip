@@ -558,23 +558,60 @@ struct kache_table
 
     void rehash_ov(uint64_t new_slots)
     {
+        check_budget(new_slots * slot_words() * 8, "growing the overflow table");
         uint64_t* nov = alloc_ov(new_slots);
-        const TableView N = view_of(buckets, cap, nov, new_slots);
-        const uint64_t blocks = (ov_slots + TILE - 1) / TILE;
-        timed("ov_rehash", [&]{
-            if(map)
-                hipLaunchKernelGGL(k_ov_rehash<true>, dim3(grid_for(k_ov_rehash<true>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
-            else
-                hipLaunchKernelGGL(k_ov_rehash<false>, dim3(grid_for(k_ov_rehash<false>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
-        });
-        const Counters c = read_counters();
-        if(c.failed)
+        try
         {
+            const TableView N = view_of(buckets, cap, nov, new_slots);
+            const uint64_t blocks = (ov_slots + TILE - 1) / TILE;
+            timed("ov_rehash", [&]{
+                if(map)
+                    hipLaunchKernelGGL(k_ov_rehash<true>, dim3(grid_for(k_ov_rehash<true>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+                else
+                    hipLaunchKernelGGL(k_ov_rehash<false>, dim3(grid_for(k_ov_rehash<false>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+            });
+            const Counters c = read_counters();
+            if(c.failed)
+                throw hip_failure("overflow rehash found no room");
+        }
+        catch(...)
+        {
+            // The old overflow table is untouched: the table stays as it was.
+            (void)hipStreamSynchronize(stream);
             (void)hipFree(nov);
-            throw hip_failure("overflow rehash found no room");
+            throw;
         }
         HIP(hipFree(ov));
         ov = nov, ov_slots = new_slots;
+    }
+
+    // --- Device-memory budget ---
+
+    // Most bytes of HBM the table may hold at once, its transient arrays
+    // during growth included (0: no limit but the device's own).
+    uint64_t hbm_budget = 0;
+
+    uint64_t device_bytes() const { return cap * bucket_bytes() + ov_slots * slot_words() * 8; }
+
+    // Throws bad_alloc (KACHE_ENOMEM) before growth that would need `more`
+    // bytes beside the table's current arrays, beyond the budget or beyond
+    // the device's free memory; nothing has changed then.
+    void check_budget(uint64_t more, const char* what) const
+    {
+        const uint64_t need = device_bytes() + more;
+        if(hbm_budget != 0 && need > hbm_budget)
+        {
+            last_error = std::string(what) + " needs " + std::to_string(need) + " bytes of HBM, beyond the table's budget of " +
+                         std::to_string(hbm_budget);
+            throw std::bad_alloc();
+        }
+        size_t free_b = 0, total_b = 0;
+        if(hipMemGetInfo(&free_b, &total_b) == hipSuccess && more > free_b)
+        {
+            last_error = std::string(what) + " needs " + std::to_string(more) + " more bytes of HBM, " + std::to_string(free_b) +
+                         " free";
+            throw std::bad_alloc();
+        }
     }
 
     // Grows the overflow table until it has room for `incoming` more keys
@@ -687,45 +724,66 @@ struct kache_table
         uint8_t* nb = nullptr;      // written whole by the split
         uint64_t* nov = nullptr;
         DevBuf away;                // one bit per old slot: the keys away from their primary bucket
-        if(pre.buckets)
+        const auto drop_new = [&]
         {
-            nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
-            zero(nov, nov_slots * slot_words() * 8);
-            away.p = pre.away, away.bytes = pre.away_bytes;
-        }
-        else
+            (void)hipStreamSynchronize(stream);
+            if(nb)
+                (void)hipFree(nb), nb = nullptr;
+            if(nov)
+                (void)hipFree(nov), nov = nullptr;
+        };
+        try
         {
-            nb = dev_alloc<uint8_t>(ncap * bucket_bytes());
-            try
+            if(pre.buckets)
             {
+                nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
+                away.p = pre.away, away.bytes = pre.away_bytes;
+                check_budget(ncap * bucket_bytes() + nov_slots * slot_words() * 8 + away.bytes, "doubling the table");
+                zero(nov, nov_slots * slot_words() * 8);
+            }
+            else
+            {
+                check_budget(ncap * bucket_bytes() + nov_slots * slot_words() * 8 + pairs * 8, "doubling the table");
+                nb = dev_alloc<uint8_t>(ncap * bucket_bytes());
                 nov = alloc_ov(nov_slots);
+                away.reserve(pairs * 8);
             }
-            catch(...)
-            {
-                (void)hipFree(nb);
-                throw;
-            }
-            away.reserve(pairs * 8);
+        }
+        catch(...)
+        {
+            drop_new();
+            throw;
         }
 
-        const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
-        const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
-        const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
-        timed("split", [&]{
-            if(map)
-                hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
-            else
-                hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
-        });
-        timed("reinsert", [&]{
-            if(map)
-                hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
-            else
-                hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
-        });
-        const Counters c = read_counters();
-        if(c.failed)
-            throw hip_failure("resize found no room in the overflow table");
+        Counters c;
+        try
+        {
+            const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
+            const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
+            const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
+            timed("split", [&]{
+                if(map)
+                    hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
+                else
+                    hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
+            });
+            timed("reinsert", [&]{
+                if(map)
+                    hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
+                else
+                    hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
+            });
+            c = read_counters();
+            if(c.failed)
+                throw hip_failure("resize found no room in the overflow table");
+        }
+        catch(...)
+        {
+            // The split and reinsert only read the old arrays: the table is
+            // unchanged, and the new arrays go.
+            drop_new();
+            throw;
+        }
 
         free_later({buckets, ov, away.p});
         away.p = nullptr, away.bytes = 0;
@@ -784,7 +842,22 @@ struct kache_table
         // the final capacity, which is the same.
         const uint64_t th = resize_th, st = step;
         for(uint32_t i = 0; i < doublings; ++i)
-            resize();
+        {
+            try
+            {
+                resize();
+            }
+            catch(...)
+            {
+                // The table keeps the capacity it has: the threshold and step
+                // are that capacity's again, and the flushed size stays at or
+                // above the threshold, so the next flush tries the doubling
+                // again (include/Streaming_Kmer_Hash_Table.hpp:2266-2286).
+                resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
+                step = step_for(resize_th);
+                throw;
+            }
+        }
         resize_th = th, step = st;
     }
 
@@ -2581,6 +2654,24 @@ int kache_amd_describe_device(const char* text, uint64_t text_bytes, uint64_t* k
     });
 }
 
+int kache_amd_hashes_device(const uint64_t* x, uint64_t n, uint64_t cap, uint64_t* out) noexcept
+{
+    if(n > 0 && (!x || !out))
+        return KACHE_EINVAL;
+    if(cap == 0 || cap > MAX_CAP || n > (uint64_t(1) << 32))
+        return KACHE_EINVAL;
+    return guard([&]()
+    {
+        if(n == 0)
+            return static_cast<int>(KACHE_OK);
+        const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((n + TILE - 1) / TILE, 4096));
+        hipLaunchKernelGGL(k_hashes, dim3(g), dim3(TILE), 0, 0, x, n, cap, out);
+        HIP(hipGetLastError());
+        HIP(hipDeviceSynchronize());
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
 int kache_amd_profile(kache_table* t, int on) noexcept
 {
     if(t == nullptr)
@@ -2611,6 +2702,15 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) noexcept
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
     t->blocks_per_cu = blocks;
+    return KACHE_OK;
+}
+
+int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) noexcept
+{
+    if(t == nullptr)
+        return KACHE_EINVAL;
+    std::lock_guard<std::mutex> lock(t->mu);
+    t->hbm_budget = bytes;
     return KACHE_OK;
 }
 

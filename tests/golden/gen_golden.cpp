@@ -1,21 +1,35 @@
-// Fixture writer for the describer and hash golden vectors.
+// Fixture writer for the describer, hash and placement golden vectors.
 //
 // Built against the reference headers (include/Kmer_Batch.hpp, include/Kmer_Lanes.hpp,
-// include/DNA_Utility.hpp, include/xxHash/xxhash.h) by `make -C oracle golden`, which
-// compiles it twice: once for AVX2 (the multiply checksum, include/Kmer_Batch.hpp:49-50)
-// and once for AVX-512 (the Wang checksum, include/Kmer_Batch.hpp:47-48).  It writes
-// plain little-endian binary records; tests/golden/README.md gives their layout.
+// include/DNA_Utility.hpp, include/xxHash/xxhash.h, include/Streaming_Kmer_Hash_Table.hpp,
+// include/Concurrent_Hash_Table.hpp) by `make -C oracle golden`, which compiles it twice:
+// once for AVX2 (the multiply checksum, include/Kmer_Batch.hpp:49-50) and once for AVX-512
+// (the Wang checksum, include/Kmer_Batch.hpp:47-48).  It writes plain little-endian binary
+// files, which tests/golden/make_golden.py packs into .npz fixtures:
+//   seqs.bin            u64 n, u64 lens[n], then the n sequences' bytes back to back
+//   describe_<tag>.bin  one 32-byte record per k-mer: u32 q, u32 i, u64 key, u64 h, u8 c, u8 fwd, 6 pad bytes
+//   scalars_<tag>.bin   u64 n, then n rows of 8 u64: x, wang_u64(x), kmer_checksum(x), canonical 62-bit x,
+//                       its minimizer hash, XXH3_64bits_withSeed(&x, 8, 0), the same with seed 2^63,
+//                       XXH3_64bits(&x, 8)
+//   placement_<tag>.bin the table's private placement helpers, called through their own code:
+//                       u64 n_cap, caps[n_cap], u64 n_h, then n_h rows of (h, filter_pos(h),
+//                       n_cap x (bucket_of(h, cap), pick_bucket(h, cap))); u64 n_tab, then per table
+//                       (u64 max_sz, u64 buckets, u64 overflow slots, u64 n_sec, n_sec x (h, p, q) of
+//                       secondaries(h)); u64 n_ov, then n_ov pairs (max_n, overflow-table capacity)
 //
 // Usage: gen_golden <out_dir> <tag>
 
 #include "DNA_Utility.hpp"
 #include "Kmer_Batch.hpp"
 #include "Kmer_Lanes.hpp"
+#include "Streaming_Kmer_Hash_Table.hpp"
+#include "Concurrent_Hash_Table.hpp"
 #include "xxHash/xxhash.h"
 
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 using namespace kache_hash;
@@ -24,6 +38,36 @@ namespace
 {
 
 constexpr uint16_t k = 31, l = 22;
+
+typedef Streaming_Kmer_Hash_Table<k, true, void, l> Set_Table;
+
+// Access to the table's private placement helpers for the fixtures: the names
+// of an explicit instantiation are not access-checked ([temp.spec]/6), so
+// Reach<Tag, &Set_Table::f> hands out a pointer to f through get(Tag).
+template <typename Tag, typename Tag::type M>
+struct Reach
+{
+    friend typename Tag::type get(Tag) { return M; }
+};
+
+struct Bucket_Of { typedef std::size_t (*type)(uint64_t, std::size_t); friend type get(Bucket_Of); };
+struct Pick_Bucket { typedef std::size_t (*type)(uint64_t, std::size_t); friend type get(Pick_Bucket); };
+struct Filter_Pos { typedef std::size_t (*type)(uint64_t); friend type get(Filter_Pos); };
+struct Secondaries { typedef std::pair<std::size_t, std::size_t> (Set_Table::*type)(uint64_t) const; friend type get(Secondaries); };
+
+template struct Reach<Bucket_Of, &Set_Table::bucket_of>;
+template struct Reach<Pick_Bucket, &Set_Table::pick_bucket>;
+template struct Reach<Filter_Pos, &Set_Table::filter_pos>;
+template struct Reach<Secondaries, &Set_Table::secondaries>;
+
+// The overflow table's capacity, through its own member.
+template <typename Tag, auto M>
+struct Reach_Auto
+{
+    friend auto get(Tag) { return M; }
+};
+struct Ov_Member { friend auto get(Ov_Member); };
+template struct Reach_Auto<Ov_Member, &Set_Table::ov>;
 
 struct SplitMix
 {
@@ -166,6 +210,56 @@ int main(const int argc, const char* const argv[])
             put<uint64_t>(f, XXH3_64bits_withSeed(&x, sizeof(x), 0));
             put<uint64_t>(f, XXH3_64bits_withSeed(&x, sizeof(x), uint64_t(1) << 63));
             put<uint64_t>(f, XXH3_64bits(&x, sizeof(x)));
+        }
+        std::fclose(f);
+    }
+
+    // 3. Placement: bucket_of, pick_bucket and filter_pos over seeded hashes and
+    //    capacities (include/Streaming_Kmer_Hash_Table.hpp:1014-1019, 1055-1068),
+    //    secondaries of live tables (:2111-2115), and overflow capacities, of
+    //    the overflow table alone (include/Concurrent_Hash_Table.hpp:158-164) and
+    //    as a table sizes it (include/Streaming_Kmer_Hash_Table.hpp:775).
+    {
+        SplitMix r{0x9A7E5ull};
+        const std::vector<uint64_t> caps{1, 2, 3, 7, 40, 64, 1000, 4095, 3906251, 78125001, (uint64_t(1) << 31) + 11,
+                                         (uint64_t(1) << 32) - 1};
+        std::vector<uint64_t> H{0, 1, 0xFFFFFFFFull, 0x100000000ull, 0xFF00000000ull, ~uint64_t(0), uint64_t(1) << 63};
+        while(H.size() < 2048)
+            H.push_back(r.next());
+
+        std::FILE* f = std::fopen((dir + "/placement_" + tag + ".bin").c_str(), "wb");
+        put<uint64_t>(f, caps.size());
+        for(const auto c : caps)
+            put<uint64_t>(f, c);
+        put<uint64_t>(f, H.size());
+        for(const auto h : H)
+        {
+            put<uint64_t>(f, h), put<uint64_t>(f, get(Filter_Pos{})(h));
+            for(const auto c : caps)
+                put<uint64_t>(f, get(Bucket_Of{})(h, c)), put<uint64_t>(f, get(Pick_Bucket{})(h, c));
+        }
+
+        const std::vector<uint64_t> sizes{10, 1000, 40000, 1 << 20};
+        put<uint64_t>(f, sizes.size());
+        for(const auto n : sizes)
+        {
+            const Set_Table t(n, 1);
+            put<uint64_t>(f, n), put<uint64_t>(f, t.capacity() / 32), put<uint64_t>(f, (t.*get(Ov_Member{}))->capacity());
+            const std::size_t n_sec = 256;
+            put<uint64_t>(f, n_sec);
+            for(std::size_t i = 0; i < n_sec; ++i)
+            {
+                const auto [p, q] = (t.*get(Secondaries{}))(H[i]);
+                put<uint64_t>(f, H[i]), put<uint64_t>(f, p), put<uint64_t>(f, q);
+            }
+        }
+
+        const std::vector<uint64_t> ov_n{1, 2, 800, 819, 820, 1023, 1024, 1025, 3906251, 5000000};
+        put<uint64_t>(f, ov_n.size());
+        for(const auto n : ov_n)
+        {
+            const Concurrent_Hash_Table<Kmer<k>, uint64_t, Hash<Kmer<k>>> ov(n);
+            put<uint64_t>(f, n), put<uint64_t>(f, ov.capacity());
         }
         std::fclose(f);
     }

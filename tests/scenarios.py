@@ -67,6 +67,24 @@ def distinct(S):
     return t.size()
 
 
+def reductions(t, w):
+    """test/capi.cpp:276-286: kache_reduce over the masks {max, 0x5A..5A & max}
+    and each reduction a table of width w has (KEYS only for a set)."""
+    mx = (1 << (8 * w)) - 1 if w else (1 << 64) - 1
+    out = []
+    for m in (mx, 0x5A5A5A5A5A5A5A5A & mx):
+        for what in range(3 if w else 1):
+            out.append(t.reduce(what, m, 1 + what))
+    return out
+
+
+def first_kmers(t, Q):
+    """test/capi.cpp:361-369: kache_get of each query's first k-mer (queries
+    of at least 31 characters), as (found, value) arrays."""
+    r = [t.get(q[:31]) for q in Q if len(q) >= 31]
+    return np.array([f for f, _ in r], np.uint8), np.array([v for _, v in r], np.uint64)
+
+
 def run(table_cls, api, s, S=None, Q=None):
     """Runs scenario `s` through a table of class `table_cls` on `api`."""
     if S is None:
@@ -91,6 +109,8 @@ def run(table_cls, api, s, S=None, Q=None):
     r["counts"] = t.count(Q)
     if w:
         r["sums"], r["masked"] = t.sum(Q), t.count_masked(Q, M_FWD, M_REV)
+    r["reduce"] = reductions(t, w)
+    r["get"] = first_kmers(t, Q)
     if w and s["update"] is not None:
         pres = []
         for b, batch in enumerate(G.ragged_batches(Q)):
@@ -150,6 +170,16 @@ class OracleTable:
     def count_masked(self, Q, mf, mr):
         m = (1 << (8 * self.width)) - 1
         return self.t.counts(Q, mf & m, mr & m)[2]
+
+    def reduce(self, what, mask, workers=1):
+        return self.t.reduce(what, mask)
+
+    def get(self, kmer):
+        import oracle as O
+        valid, key, _, _, _ = O.describe(bytes(kmer[:31]))
+        if not valid[0]:
+            return False, 0
+        return self.t.get(int(key[0]))
 
     def close(self):
         pass

@@ -119,3 +119,32 @@ def test_malformed_inputs_fail(api, tmp_path, what, text):
 def test_absent_file_fails(api, tmp_path):
     with pytest.raises(capi.KacheError):
         capi.read_fastx(api, str(tmp_path / "absent"))
+
+
+@pytest.mark.parametrize("kind", ["fasta", "fastq"])
+def test_reader_named_pipe(api, tmp_path, kind):
+    """Input from a named pipe, written in small pieces by another thread while
+    the reader reads (as test/counter.cpp:198-225 feeds the reference's
+    reader): the same sequences as from a regular file."""
+    import threading
+    import time
+    rng = np.random.default_rng(3)
+    R = records(rng, 2000, 500, 50_000)
+    text = fasta(R, rng, 70, b"\n") if kind == "fasta" else fastq(R, rng, b"\r\n")
+    p = tmp_path / ("in." + kind)
+    os.mkfifo(p)
+
+    def writer():
+        with open(p, "wb", buffering=0) as f:
+            for i in range(0, len(text), 65_537):
+                f.write(text[i:i + 65_537])
+                if i % (16 * 65_537) == 0:
+                    time.sleep(0.001)
+
+    th = threading.Thread(target=writer)
+    th.start()
+    try:
+        got = capi.read_fastx(api, str(p))
+    finally:
+        th.join(timeout=60)
+    assert got == expected(R)

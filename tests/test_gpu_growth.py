@@ -1,0 +1,72 @@
+"""Growth under an HBM budget (kache_amd_set_hbm_budget): a doubling that
+would pass the budget fails before it allocates, with KACHE_ENOMEM; the
+call's keys stay applied, the table keeps its capacity and every result stays
+equal to the oracle's; once the budget allows it, the next call doubles."""
+import numpy as np
+import pytest
+
+import kache_hash_amd as KA
+import oracle as O
+import seqgen as G
+from kache_hash_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    return KA.api()
+
+
+def same_as_oracle(t, ot, queries):
+    k, v, _ = t.for_each()
+    ko, vo = ot.dump()
+    assert np.array_equal(k, ko) and np.array_equal(v, vo)
+    f, val, fw = t.lookup(queries)
+    f2, val2, fw2 = ot.lookup(queries)
+    assert (f == f2).all() and (fw == fw2).all()
+    if t.width:
+        assert (val == val2).all()
+
+
+@pytest.mark.parametrize("width", [0, 4])
+def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
+    g = G.genome(300_000, 91 + width)
+    reads = G.reads(g, 12_000, 150, 92, sub=0.01, nrate=0.001, lower=0.1)
+    queries = G.reads(g, 500, 150, 93, sub=0.01) + G.pieces(G.genome(5_000, 94), 200)
+    print(f"growth budget, width {width}: {len(reads)} reads in calls of 1000", flush=True)
+    t = capi.Table(api, width, expected_keys=20_000)
+    ot = O.OTable(width)
+    u = t.register()
+    cap0 = t.stats()["capacity"]
+    t.set_hbm_budget(t.info()["device_bytes"] + (64 << 10))     # far below what a doubling needs
+    refused = None
+    for i in range(0, len(reads), 1000):
+        part = reads[i:i + 1000]
+        expect = ot.insert(part, capi.ADD, 1)
+        try:
+            assert t.insert(part, capi.ADD, 1, u) == expect
+        except capi.KacheError as e:
+            assert e.status == capi.KACHE_ENOMEM
+            assert "budget" in api.last_error()
+            refused = i
+            break
+    assert refused is not None, "the table never needed to double"
+    # The refused call's keys are in; capacity and results are exact.
+    assert t.stats()["capacity"] == cap0 and t.info()["resizes"] == 0
+    assert t.size() == ot.size()
+    same_as_oracle(t, ot, queries)
+    # Still refused while the budget holds; the keys of that call are applied too.
+    part = reads[refused + 1000:refused + 2000]
+    ot.insert(part, capi.ADD, 1)
+    with pytest.raises(capi.KacheError) as e:
+        t.insert(part, capi.ADD, 1, u)
+    assert e.value.status == capi.KACHE_ENOMEM
+    same_as_oracle(t, ot, queries)
+    # Without the budget the next call doubles, and the table stays exact.
+    t.set_hbm_budget(0)
+    part = reads[refused + 2000:refused + 3000]
+    assert t.insert(part, capi.ADD, 1, u) == ot.insert(part, capi.ADD, 1)
+    assert t.stats()["capacity"] >= 2 * cap0 and t.info()["resizes"] >= 1
+    same_as_oracle(t, ot, queries)
+    t.close()

@@ -53,6 +53,40 @@ def test_device_describer_matches_reference_vectors(api):
     assert (c[pos] == d["c_wang"]).all() and (fwd[pos] == d["fwd"]).all()
 
 
+def device_hashes(api, x, cap):
+    """kache_amd_hashes_device over x: one row of 10 words per input."""
+    x = np.ascontiguousarray(x, np.uint64)
+    bx, bo = capi.DeviceBuffer(api, 8 * len(x)), capi.DeviceBuffer(api, 80 * len(x))
+    bx.upload(x)
+    api.check(api.lib.kache_amd_hashes_device(bx.ptr, len(x), cap, bo.ptr), "kache_amd_hashes_device")
+    return bo.download(dtype=np.uint64).reshape(-1, 10)
+
+
+def test_device_hashes_match_reference_vectors(api):
+    """The device builds of the Wang hash, the checksum, the minimizer hash,
+    XXH3's 8-byte path with both seeds, bucket_of, filter_pos, pick_bucket
+    and the secondaries, bit for bit against vectors the reference's own code
+    wrote (tests/golden/gen_golden.cpp)."""
+    s = np.load(os.path.join(GOLD, "scalars.npz"), allow_pickle=False)
+    p = np.load(os.path.join(GOLD, "placement.npz"), allow_pickle=False)
+    print("device hashes:", len(s["x"]), "scalars,", len(p["h"]), "hashes x", len(p["caps"]), "capacities", flush=True)
+    o = device_hashes(api, s["x"], 1000)
+    assert (o[:, 0] == s["wang"]).all()
+    assert (o[:, 1] == s["c_wang"]).all()
+    assert (o[:, 2] == s["minimizer"]).all()
+    assert (o[:, 3] == s["xxh3_s0"]).all() and (o[:, 3] == s["xxh3"]).all()
+    assert (o[:, 4] == s["xxh3_s63"]).all()
+    for j, cap in enumerate(p["caps"]):
+        o = device_hashes(api, p["h"], int(cap))
+        assert (o[:, 5] == p["bucket_of"][:, j]).all(), int(cap)
+        assert (o[:, 6] == p["filter_pos"]).all()
+        assert (o[:, 7] == p["pick_bucket"][:, j]).all(), int(cap)
+    for t in range(len(p["tab_n"])):
+        sec = p["sec"][t]
+        o = device_hashes(api, sec[:, 0], int(p["tab_buckets"][t]))
+        assert (o[:, 8] == sec[:, 1]).all() and (o[:, 9] == sec[:, 2]).all()
+
+
 GOLDEN = {g["name"]: g for g in json.load(open(os.path.join(GOLD, "ref_tables.json")))["scenarios"]}
 
 
@@ -75,6 +109,9 @@ def test_scenario_matches_oracle_and_reference(api, name):
     assert (a["counts"] == b["counts"]).all() and digest(a["counts"]) == g["counts"]
     if s["width"]:
         assert (a["sums"] == b["sums"]).all() and (a["masked"] == b["masked"]).all()
+    assert a["reduce"] == b["reduce"] == g["reduce"]
+    assert (a["get"][0] == b["get"][0]).all() and (a["get"][1] == b["get"][1]).all()
+    assert digest(*a["get"]) == g["get"]
     if g["present"] is not None:
         assert a["present"] == b["present"] == g["present"]
         assert (a["values2"] == b["values2"]).all() and digest(a["keys2"], a["values2"]) == g["dump_after_update"]

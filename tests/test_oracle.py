@@ -129,20 +129,51 @@ def test_sizing_known_answers():
         assert c == 1 or int((c - 1) * 32 * 0.8) <= n
 
 
-def test_bucket_functions():
-    rng = np.random.default_rng(9)
-    for h in rng.integers(0, 2**63, 500, dtype=np.uint64):
-        h = int(h) * 2 + 1
-        for cap in (1, 7, 1000, 3906251, (1 << 32) - 1):
-            b = O.bucket_of(h, cap)
-            assert b == ((h & 0xFFFFFFFF) * cap) >> 32 and b < cap
-            if 2 * cap <= (1 << 32) - 1:    # doubling stays within the most buckets
-                assert O.bucket_of(h, 2 * cap) in (2 * b, 2 * b + 1)
-            p, q = O.secondaries(h, cap)
-            assert p == (xxhash.xxh3_64_intdigest(h.to_bytes(8, "little")) * cap) >> 64
-            assert q == (xxhash.xxh3_64_intdigest(h.to_bytes(8, "little"), seed=1 << 63) * cap) >> 64
-        x = (h >> 32) & 0xFF
-        assert O.filter_pos(h) == (x if x else 1)
+def test_placement_against_reference_vectors():
+    """bucket_of, filter_pos and pick_bucket as the reference's own (private)
+    helpers compute them, over seeded hashes and capacities from 1 to 2^32 - 1
+    (include/Streaming_Kmer_Hash_Table.hpp:1014-1019, 1055-1068; written by
+    tests/golden/gen_golden.cpp)."""
+    p = load("placement.npz")
+    caps = [int(c) for c in p["caps"]]
+    for i in range(len(p["h"])):
+        h = int(p["h"][i])
+        assert O.filter_pos(h) == int(p["filter_pos"][i])
+        for j, cap in enumerate(caps):
+            assert O.bucket_of(h, cap) == int(p["bucket_of"][i, j])
+            assert O.pick_bucket(h, cap) == int(p["pick_bucket"][i, j])
+
+
+def test_secondaries_and_overflow_sizes_against_reference_tables():
+    """Secondaries of live reference tables (:2111-2115), their bucket counts
+    (:1003-1011) and the overflow capacities the reference gives an overflow
+    table alone (include/Concurrent_Hash_Table.hpp:158-164) and as a table
+    sizes it (include/Streaming_Kmer_Hash_Table.hpp:775)."""
+    p = load("placement.npz")
+    for n, slots in zip(p["ov_n"], p["ov_slots"]):
+        assert O.ov_slots_for(int(n)) == int(slots)
+    for t in range(len(p["tab_n"])):
+        cap = O.cap_for(int(p["tab_n"][t]), 0.8, False)
+        assert cap == int(p["tab_buckets"][t])
+        assert O.ov_slots_for(max(1024, cap)) == int(p["tab_ov_slots"][t])
+        for h, ps, qs in p["sec"][t]:
+            assert O.secondaries(int(h), cap) == (int(ps), int(qs))
+
+
+def test_placement_doubling_and_planted_mistakes():
+    """Doubling maps bucket b to 2b or 2b + 1 (the property the in-order
+    split relies on); a transposed multiply or a dropped mask fails the
+    reference vectors."""
+    p = load("placement.npz")
+    h = [int(x) for x in p["h"][:200]]
+    for x in h:
+        for cap in (1, 7, 1000, 3906251, (1 << 31) - 1):
+            b = O.bucket_of(x, cap)
+            assert b < cap and O.bucket_of(x, 2 * cap) in (2 * b, 2 * b + 1)
+    j = [int(c) for c in p["caps"]].index(3906251)
+    assert any((((x >> 32) * 3906251) >> 32) != int(p["bucket_of"][i, j]) for i, x in enumerate(h))
+    assert any(((x * 3906251) >> 32) & ((1 << 64) - 1) != int(p["bucket_of"][i, j]) for i, x in enumerate(h))
+    assert any(((x >> 32) & 0xFF) != int(p["filter_pos"][i]) for i, x in enumerate(h))    # the 0 -> 1 rule
 
 
 def test_schedule_plain_cases():
@@ -174,6 +205,8 @@ def test_oracle_matches_reference_tables(name):
     assert digest(r["keys"], r["values"]) == gold["dump"]
     assert digest(*r["lookup"]) == gold["lookup"]
     assert digest(r["counts"]) == gold["counts"]
+    assert r["reduce"] == gold["reduce"]
+    assert digest(*r["get"]) == gold["get"]
     if gold["present"] is not None:
         assert r["present"] == gold["present"]
         assert digest(r["keys2"], r["values2"]) == gold["dump_after_update"]
@@ -209,3 +242,5 @@ def test_oracle_matches_reference_library_live():
         assert (x == y).all()
     assert (a["counts"] == b["counts"]).all() and (a["sums"] == b["sums"]).all() and (a["masked"] == b["masked"]).all()
     assert a["present"] == b["present"] and (a["values2"] == b["values2"]).all()
+    assert a["reduce"] == b["reduce"]
+    assert (a["get"][0] == b["get"][0]).all() and (a["get"][1] == b["get"][1]).all()
