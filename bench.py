@@ -56,8 +56,9 @@ def random_line_roof(kmers_per_s):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=None, help="timed batches (default: all after warmup)")
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed steps (count map: full passes over all reads, default 5; other workloads: their own)")
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--reads", type=int, default=20_000_000)
     p.add_argument("--batch", type=int, default=65_536)
     p.add_argument("--genome", type=int, default=100_000_000)
@@ -102,10 +103,33 @@ def valid_positions(host_rows):
     return (win == 0).sum(axis=1)
 
 
-def cpu_baseline(host_text, n_reads, expected, threads):
+def host_cores():
+    """Threads for the CPU baselines: every core this process may run on (its
+    affinity set, within the cgroup's CPU quota when one is set), at most the
+    reference's 128 users (include/Streaming_Kmer_Hash_Table.hpp:59); with the
+    figures they come from."""
+    cpus = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = cpus
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    usable = avail if quota is None else max(1, min(avail, int(quota)))
+    return min(128, usable), {"host_cpus": cpus, "affinity_cpus": avail, "cgroup_cpu_quota": quota}
+
+
+def cpu_baseline(host_text, n_reads, expected, threads, look_text=None, look_reads=0):
     """The reference's C ABI (oracle/_ref, built from its sources) inserting
     the same reads from `threads` threads, each its own user, in calls of
-    1,024 reads; timed over the inserts only."""
+    1,024 reads (timed over the inserts only); then, on the table built,
+    kache_count of the lookup reads from the same threads in calls of 1,024
+    reads (timed over the lookups only)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from kache_hash_amd import capi
@@ -117,35 +141,58 @@ def cpu_baseline(host_text, n_reads, expected, threads):
     users = [t.register() for _ in range(threads)]
     L1 = READ_LEN + 1
     per_call = 1024
-    calls = [(a, min(per_call, n_reads - a)) for a in range(0, n_reads, per_call)]
-    base = host_text.ctypes.data
     lens = np.full(per_call, READ_LEN, dtype=np.uint64)
-    ptr_tabs = [(np.arange(c, dtype=np.uint64) * L1 + np.uint64(base + a * L1)).astype(np.uint64) for a, c in calls]
+
+    def calls_of(text, n):
+        base = text.ctypes.data
+        calls = [(a, min(per_call, n - a)) for a in range(0, n, per_call)]
+        return calls, [(np.arange(c, dtype=np.uint64) * L1 + np.uint64(base + a * L1)).astype(np.uint64) for a, c in calls]
+
+    def parallel(work):
+        th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+        t0 = time.perf_counter()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        return time.perf_counter() - t0
+
+    calls, ptr_tabs = calls_of(host_text, n_reads)
     totals = [0] * threads
 
-    def work(i):
+    def insert(i):
         n = C.c_uint64()
         for j in range(i, len(calls), threads):
-            a, c = calls[j]
-            st = api.lib.kache_insert(t.h, users[i], ptr_tabs[j].ctypes.data, lens.ctypes.data, c, capi.ADD, 1, C.byref(n))
+            st = api.lib.kache_insert(t.h, users[i], ptr_tabs[j].ctypes.data, lens.ctypes.data, calls[j][1], capi.ADD, 1, C.byref(n))
             assert st == 0
             totals[i] += n.value
 
-    th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
-    t0 = time.perf_counter()
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
-    dt = time.perf_counter() - t0
+    dt = parallel(insert)
     rows = host_text[: n_reads * L1].reshape(n_reads, L1)[:, :READ_LEN]
     kmers = int(valid_positions(rows).sum())
     size = t.size()
+    out = {"value": kmers / dt, "unit": "k-mer upserts/s", "cores": threads, "kind": "reference",
+           "sample": f"all {n_reads} reads ({kmers} k-mers, {size} distinct) upserted into a fresh 100M-key u64 map "
+                     f"by {threads} threads, one user each, in calls of {per_call} reads; {os.path.basename(path)}",
+           "seconds": dt, "resizes_capacity": t.stats()["capacity"]}
+    if look_text is not None and look_reads:
+        lcalls, lptrs = calls_of(look_text, look_reads)
+        hits = [0] * threads
+
+        def lookup(i):
+            cnt = np.zeros(per_call, np.uint64)
+            for j in range(i, len(lcalls), threads):
+                st = api.lib.kache_count(t.h, lptrs[j].ctypes.data, lens.ctypes.data, lcalls[j][1], cnt.ctypes.data)
+                assert st == 0
+                hits[i] += int(cnt[:lcalls[j][1]].sum())
+
+        dtl = parallel(lookup)
+        lk = int(valid_positions(look_text[: look_reads * L1].reshape(look_reads, L1)[:, :READ_LEN]).sum())
+        out.update(lookups_per_s=lk / dtl, lookup_kmers=lk, lookup_hits=sum(hits), lookup_seconds=dtl,
+                   lookup_sample=f"kache_count of {look_reads} fresh reads ({lk} k-mers) by {threads} threads in calls of "
+                                 f"{per_call} reads, on the table built")
     t.close()
-    return {"value": kmers / dt, "unit": "k-mer upserts/s", "cores": threads, "kind": "reference",
-            "sample": f"first {n_reads} reads ({kmers} k-mers, {size} distinct) upserted into a fresh 100M-key u64 map "
-                      f"by {threads} threads in calls of {per_call} reads; {os.path.basename(path)}",
-            "seconds": dt}
+    return out
 
 
 def main():
@@ -167,10 +214,9 @@ def main():
         return batched_main(a, KA, capi, api)
     if a.seed is None:
         a.seed = 0xC2
-    nb = (a.reads + a.batch - 1) // a.batch
-    steps = a.steps if a.steps is not None else max(nb - a.warmup, 1)
-    need = a.warmup + steps
-    n_reads = min(a.reads, need * a.batch)
+    steps = a.steps if a.steps is not None else 5
+    warmup = a.warmup
+    n_reads = a.reads
     t_gen = time.perf_counter()
     # Reads come straight from the seeded genome's hash (the genome itself is
     # never materialized: kache_amd_synth_reads derives each window from it).
@@ -178,51 +224,57 @@ def main():
     look = gen_text(api, capi, a.lookup_reads, a.seed, a.genome, a.seed + 2)
     print(f"bench: generated {n_reads} reads + {a.lookup_reads} lookup reads in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
 
-    t = KA.table(width=8, expected_keys=a.expected)
-    if a.blocks_per_cu:
-        t.set_blocks_per_cu(a.blocks_per_cu)
-    u = t.register()
     L1 = READ_LEN + 1
     base = text.ptr
+    # Batch bounds of the whole run: 65,536 reads each (the last one shorter).
+    edges = np.array([min(r, n_reads) * L1 for r in range(0, n_reads + a.batch, a.batch)], np.uint64)
+    edges = np.unique(edges)
+    n_batches = len(edges) - 1
 
-    def bounds(first, count):
-        """Batch bounds of `count` batches from batch `first` (wrapping around the reads)."""
-        out = []
-        for i in range(first, first + count):
-            r0 = (i * a.batch) % n_reads
-            c = min(a.batch, n_reads - r0)
-            out.append((r0 * L1, (r0 + c) * L1))
-        return out
+    def one_pass(profile):
+        """One step: a fresh table sized for the expected keys, every batch
+        upserted (+1, new keys at 1) as one pipelined call; returns the table."""
+        t = KA.table(width=8, expected_keys=a.expected)
+        if a.blocks_per_cu:
+            t.set_blocks_per_cu(a.blocks_per_cu)
+        t.profile(profile)
+        u = t.register()
+        t.insert_batches(base, edges, KA.ADD, 1, u)
+        return t, u
 
-    def run(first, count):
-        # Consecutive batches go in one pipelined call; a wrap-around restarts it.
-        b = bounds(first, count)
-        i = 0
-        while i < len(b):
-            j = i + 1
-            while j < len(b) and b[j][0] == b[j - 1][1]:
-                j += 1
-            edges = np.array([b[i][0]] + [e for _, e in b[i:j]], np.uint64)
-            t.insert_batches(base, edges, KA.ADD, 1, u)
-            i = j
+    # Warm-up passes (untimed); the last one also counts the k-mer positions
+    # of the reads (every one is present after the inserts).
+    kmers_per_pass = None
+    for i in range(max(warmup, 1)):
+        t, u = one_pass(False)
+        if kmers_per_pass is None:
+            kmers_per_pass = t.count_device(base, n_reads * L1, positions=True)[1]
+        t.close()
+    print(f"bench: {n_batches} batches, {kmers_per_pass} k-mer positions per pass; timing {steps} passes", file=sys.stderr, flush=True)
 
-    run(0, a.warmup)
-    t.profile(True)
+    ins_ms = split_ms = re_ms = 0.0
+    ins_launches = 0
+    api.check(api.lib.kache_amd_device_sync(), "sync")
     t0 = time.perf_counter()
-    run(a.warmup, steps)
+    for i in range(steps):
+        t, u = one_pass(True)
+        m, n = t.kernel_time("insert")
+        ins_ms, ins_launches = ins_ms + m, ins_launches + n
+        split_ms += t.kernel_time("split")[0]
+        re_ms += t.kernel_time("reinsert")[0]
+        if i + 1 < steps:
+            t.close()
+    api.check(api.lib.kache_amd_device_sync(), "sync")
     dt = time.perf_counter() - t0    # each call returns after its device work
-    ins_ms, ins_launches = t.kernel_time("insert")
-    split_ms, _ = t.kernel_time("split")
-    re_ms, _ = t.kernel_time("reinsert")
     t.profile(False)
+    info = t.info()
+    stats = t.stats()
+    timed_kmers = kmers_per_pass * steps
 
-    # k-mer positions of the timed steps: every one is present after the inserts.
-    r0, r1 = a.warmup * a.batch, min(n_reads, (a.warmup + steps) * a.batch)
-    timed_kmers = t.count_device(base + r0 * L1, (r1 - r0) * L1, positions=True)[1] if r1 > r0 else 0
-
-    # Lookups: 2M fresh reads, in batches of 65,536, back to back.
-    lb = np.array([min(r, a.lookup_reads) * L1 for r in range(0, a.lookup_reads + a.batch, a.batch)], np.uint64)
-    lb = np.unique(lb)
+    # Lookups: 2M fresh reads, in batches of 65,536, back to back, on the
+    # last pass's table.
+    lb = np.unique(np.array([min(r, a.lookup_reads) * L1 for r in range(0, a.lookup_reads + a.batch, a.batch)], np.uint64))
+    t.count_batches(look.ptr, lb)       # untimed warm-up of the query path
     t.profile(True)
     t1 = time.perf_counter()
     h, p = t.count_batches(look.ptr, lb)
@@ -230,8 +282,6 @@ def main():
     hits, look_kmers = int(h.sum()), int(p.sum())
     q_ms, q_launches = t.kernel_time("query")
     t.profile(False)
-    info = t.info()
-    stats = t.stats()
 
     per_launch_kmers = timed_kmers / max(ins_launches, 1)
     avg_ms = ins_ms / max(ins_launches, 1)
@@ -245,30 +295,41 @@ def main():
             traffic = None
 
     served = served_probe(a, KA, capi, api, t, u) if a.served_requests > 0 else None
+    t.close()
 
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads, cores = host_cores()
+        if a.cpu_threads:
+            threads = a.cpu_threads
         m = min(a.cpu_sample_reads, n_reads)
         host = text.download(m * L1)
-        print(f"bench: reference CPU path on {m} reads, {threads} threads", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(host, m, a.expected, threads)
+        lhost = look.download(a.lookup_reads * L1)
+        print(f"bench: reference CPU path on {m} reads + {a.lookup_reads} lookup reads, {threads} threads ({cores})",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(host, m, a.expected, threads, lhost, a.lookup_reads)
+        if cpu:
+            cpu.update(cores)
 
     value = timed_kmers / dt
     out = {
         "metric": "k-mer upserts/s + lookups/s (k=31 count map); p50/p99 served-batch latency",
-        "value": value, "unit": "k-mer upserts/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "value": value, "unit": "k-mer upserts/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": dt * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic (seeded on device)",
-        "config": {"workload": "count map <31,u64,22>: 100 Mbp genome, 150-bp reads (0.5% subst, 0.05% N, mixed case), "
-                               "upsert +1 in batches of 65,536 reads, lookups of 2M fresh reads",
-                   "reads": n_reads, "batch_reads": a.batch, "expected_keys": a.expected, "genome_bp": a.genome},
+        "config": {"workload": "count map <31,u64,22>: 100 Mbp genome, 20M 150-bp reads (0.5% subst, 0.05% N, mixed case), "
+                               "upsert +1 in batches of 65,536 reads into a fresh 100M-key table per step (all doublings "
+                               "included), lookups of 2M fresh reads",
+                   "step": "one full pass: table creation, every batch, the doublings, table release",
+                   "reads": n_reads, "batches": n_batches, "batch_reads": a.batch, "expected_keys": a.expected,
+                   "genome_bp": a.genome},
         "lookups_per_s": look_kmers / dt_look,
         "lookup_hits": hits, "lookup_kmers": look_kmers,
-        "timed_kmers": timed_kmers,
+        "timed_kmers": timed_kmers, "kmers_per_step": kmers_per_pass,
         "keys": stats["keys"], "capacity": stats["capacity"], "resizes": info["resizes"],
         "placement_pct": [stats["primary_pct"], stats["secondary_pct"], stats["overflow_pct"]],
-        "kernel_ms": {"insert": ins_ms, "split": split_ms, "reinsert": re_ms, "query": q_ms},
+        "kernel_ms": {"insert": ins_ms, "split": split_ms, "reinsert": re_ms, "query": q_ms,
+                      "insert_launches": ins_launches, "query_launches": q_launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "k_insert<map, ADD>", "alg_bytes_per_kmer": ALG_BYTES_PER_KMER,
@@ -279,6 +340,8 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
+        if cpu.get("lookups_per_s"):
+            out["lookup_speedup_vs_cpu"] = out["lookups_per_s"] / cpu["lookups_per_s"]
     print(json.dumps(out), flush=True)
 
 
@@ -286,9 +349,10 @@ def served_probe(a, KA, capi, api, t, u):
     """Served-batch latency on the count map just built: requests of the
     serving workload's mix and shapes drawn from the same genome (count,
     lookup, update +1, upsert +1), run by the serving loop and by plain
-    per-request calls, closed loop and at Poisson arrivals of half the plain
-    path's capacity (two alternating runs of each, latencies pooled).  After
-    the timed region; its upserts change the table."""
+    per-request calls, closed loop and at Poisson arrivals of a quarter, half
+    and three quarters of the plain path's capacity (two alternating runs of
+    each per rate, latencies pooled).  After the timed region; its upserts
+    change the table."""
     g = capi.DeviceBuffer(api, a.genome)
     api.check(api.lib.kache_amd_synth_genome(g.ptr, a.genome, a.seed), "synth_genome")
     genome = g.download(a.genome)
@@ -303,25 +367,31 @@ def served_probe(a, KA, capi, api, t, u):
     ro = [m for m, (k, _) in zip(made, reqs) if k in (capi.REQ_COUNT, capi.REQ_LOOKUP)]
     cap_s = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20, ro)
     cap_p = serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1, ro)
-    rate = 0.5 * cap_p["requests_per_s"]
-    arr = np.cumsum(np.random.default_rng(a.seed + 6).exponential(1.0 / rate, len(reqs)))
-    serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro)       # untimed open-loop warm-up of each
-    serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro)
-    runs_s, runs_p = [], []
-    for _ in range(2):      # alternating, on the same arrivals; latencies pooled
-        runs_s.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
-        runs_p.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro))
-    at_s, at_p = pooled(runs_s), pooled(runs_p)
-    print(f"bench: served {cap_s['requests_per_s']:.0f} req/s vs plain {cap_p['requests_per_s']:.0f} req/s; at {rate:.0f} req/s "
-          f"served p50 {at_s['p50_ms']:.2f} p99 {at_s['p99_ms']:.2f} ms, plain p50 {at_p['p50_ms']:.2f} p99 {at_p['p99_ms']:.2f} ms",
-          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(a.seed + 6)
+    rows = []
+    for i, frac in enumerate((0.25, 0.5, 0.75)):
+        rate = frac * cap_p["requests_per_s"]
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        if i == 0:
+            serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro)       # untimed open-loop warm-up of each
+            serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro)
+        runs_s, runs_p = [], []
+        for _ in range(2):      # alternating, on the same arrivals; latencies pooled
+            runs_s.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
+            runs_p.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro))
+        at_s, at_p = pooled(runs_s), pooled(runs_p)
+        print(f"bench: served {cap_s['requests_per_s']:.0f} req/s vs plain {cap_p['requests_per_s']:.0f} req/s; at {rate:.0f} req/s "
+              f"served p50 {at_s['p50_ms']:.3f} p99 {at_s['p99_ms']:.3f} ms, plain p50 {at_p['p50_ms']:.3f} p99 {at_p['p99_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+        rows.append({"rate_req_per_s": rate, "of_plain_capacity": frac, "served_p50_ms": at_s["p50_ms"],
+                     "served_p99_ms": at_s["p99_ms"], "plain_p50_ms": at_p["p50_ms"], "plain_p99_ms": at_p["p99_ms"],
+                     "served_req_per_s": at_s["requests_per_s"], "plain_req_per_s": at_p["requests_per_s"]})
     return {"requests": len(reqs), "kmer_positions": sum(b.positions() for _, b in reqs),
             "mix": "35% count, 35% lookup, 5% update +1, 25% upsert +1; 1-10,000 sequences, lognormal median 300 bp",
             "closed_loop": {"served_req_per_s": cap_s["requests_per_s"], "served_kmers_per_s": cap_s["kmers_per_s"],
                             "plain_req_per_s": cap_p["requests_per_s"], "plain_kmers_per_s": cap_p["kmers_per_s"],
                             "served_p50_ms": cap_s["p50_ms"], "served_p99_ms": cap_s["p99_ms"]},
-            "poisson": {"rate_req_per_s": rate, "served_p50_ms": at_s["p50_ms"], "served_p99_ms": at_s["p99_ms"],
-                        "plain_p50_ms": at_p["p50_ms"], "plain_p99_ms": at_p["p99_ms"]}}
+            "poisson": rows}
 
 
 # --- Serving workload ------------------------------------------------------
@@ -563,7 +633,7 @@ def serve_main(a, KA, capi, api):
     info = t.info()
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or host_cores()[0]
         creqs = serve_requests(capi, genome, a.cpu_requests, seed + 4, a.cpu_genome)
         print(f"serve: reference CPU path, {len(creqs)} requests, {threads} threads", file=sys.stderr, flush=True)
         cpu = cpu_serve(capi, creqs, genome, a.cpu_genome, threads)
@@ -723,7 +793,7 @@ def counter_main(a, KA, capi, api):
     stats = t.stats()
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or host_cores()[0]
         print(f"counter: reference CPU path, {threads} threads", file=sys.stderr, flush=True)
         cpu = counter_cpu(capi, prime, reads, len(prime), 1 << 30, threads)
     # The update kernel's roofline: 1.25 B of input + the 64-byte metadata line per
@@ -871,7 +941,7 @@ def batched_main(a, KA, capi, api):
         t.close()
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or host_cores()[0]
         host = text.download(min(nbytes, a.cpu_sample_bytes))
         host = host[:int(np.flatnonzero(host == 10)[-1]) + 1]
         print(f"batched: reference CPU path on {len(host) >> 20} MiB, {threads} threads", file=sys.stderr, flush=True)

@@ -153,6 +153,15 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
  * tries the doubling again. */
 int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
 
+/* Bucket stores, overflow tables and doubling arrays that tables free are kept
+ * for reuse by later tables and growth on the same device (the driver wipes
+ * memory it gets back before handing it out again, which for tables of tens
+ * of gigabytes costs more than building them): at most half of the device's
+ * memory, released when an allocation finds the device full.  This releases
+ * the current device's kept arrays now; `released` (may be NULL) receives
+ * their bytes. */
+int kache_amd_trim_cache(uint64_t* released) KACHE_NOEXCEPT;
+
 /* Launch shape of the streaming inserts of kache_amd_insert_device and
  * kache_amd_insert_oriented_device: blocks per compute unit (0: as many as
  * fit), positions per launch (2^launch_log2, 16..30), and whether a launch
@@ -202,6 +211,8 @@ int kache_amd_device_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
 int kache_amd_device_free(void* ptr) KACHE_NOEXCEPT;
 int kache_amd_copy(void* dst, const void* src, uint64_t bytes, int to_device) KACHE_NOEXCEPT;
 int kache_amd_device_fill(void* ptr, uint8_t value, uint64_t bytes) KACHE_NOEXCEPT;
+/* Waits for all work on the current device (hipDeviceSynchronize). */
+int kache_amd_device_sync(void) KACHE_NOEXCEPT;
 /* Pinned host memory (hipHostMalloc): copies from and to it run at full link speed. */
 int kache_amd_host_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
 int kache_amd_host_free(void* ptr) KACHE_NOEXCEPT;

@@ -103,6 +103,7 @@ AMD_SYMBOLS = {
     "kache_amd_device_free": (C.c_int, [_P]),
     "kache_amd_copy": (C.c_int, [_P, _P, C.c_uint64, C.c_int]),
     "kache_amd_device_fill": (C.c_int, [_P, C.c_uint8, C.c_uint64]),
+    "kache_amd_device_sync": (C.c_int, []),
     "kache_amd_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(_P)]),
     "kache_amd_host_free": (C.c_int, [_P]),
     "kache_amd_lookup_groups": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P, C.c_size_t, _P, _P, _P]),
@@ -133,6 +134,7 @@ AMD_SYMBOLS = {
     "kache_amd_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kache_amd_set_blocks_per_cu": (C.c_int, [_P, C.c_uint32]),
     "kache_amd_set_hbm_budget": (C.c_int, [_P, C.c_uint64]),
+    "kache_amd_trim_cache": (C.c_int, [_P]),
     "kache_amd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
     "kache_amd_last_error": (C.c_char_p, []),
     "kache_amd_server_new": (C.c_int, [_P, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(_P)]),

@@ -196,6 +196,12 @@ int kache_amd_device_fill(void* ptr, uint8_t value, uint64_t bytes) noexcept
     return e == hipSuccess ? KACHE_OK : fail(e);
 }
 
+int kache_amd_device_sync(void) noexcept
+{
+    const hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? KACHE_OK : fail(e);
+}
+
 int kache_amd_synth_genome(char* out, uint64_t n, uint64_t seed) noexcept
 {
     if(n > 0 && out == nullptr)

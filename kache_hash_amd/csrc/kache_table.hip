@@ -183,6 +183,122 @@ struct HostBuf
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Large device arrays (bucket stores, overflow tables, doubling bitmaps) are
+// kept when a table frees them and handed out again for an array of the same
+// size on the same device, so a new table of the same shape, and the same
+// growth steps, reuse them.  The driver wipes device memory it gets back
+// before it hands it out again; at tens of gigabytes per table that wipe, not
+// the table's own work, would bound a program that makes and drops tables.
+// The cache holds at most half of a device's memory; an allocation that finds
+// the device full releases the device's cached arrays first, and
+// kache_amd_trim_cache releases them on request.  Arrays are handed back only
+// after the work that used them has ended (their table's stream synchronized).
+class ArrayCache
+{
+    std::mutex mu;
+    std::multimap<std::pair<int, uint64_t>, void*> kept;
+    std::map<int, uint64_t> kept_bytes, limits;
+    static constexpr uint64_t GRAIN = uint64_t(2) << 20;
+
+    static uint64_t rounded(uint64_t b) { return (std::max<uint64_t>(b, 1) + GRAIN - 1) / GRAIN * GRAIN; }
+
+    uint64_t limit(int dev)
+    {
+        auto it = limits.find(dev);
+        if(it != limits.end())
+            return it->second;
+        size_t total = 0;
+        const uint64_t l = hipDeviceTotalMem(&total, dev) == hipSuccess ? total / 2 : 0;
+        limits[dev] = l;
+        return l;
+    }
+
+public:
+    static ArrayCache& get()
+    {
+        static ArrayCache* c = new ArrayCache;     // lives as long as the process: no frees after the runtime's teardown
+        return *c;
+    }
+
+    // An array of at least `bytes` on the current device, which must be `dev`.
+    void* take(int dev, uint64_t bytes)
+    {
+        const uint64_t r = rounded(bytes);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const auto it = kept.find({dev, r});
+            if(it != kept.end())
+            {
+                void* p = it->second;
+                kept.erase(it);
+                kept_bytes[dev] -= r;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, r);
+        if((e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) && trim(dev) > 0)
+        {
+            (void)hipGetLastError();
+            e = hipMalloc(&p, r);
+        }
+        check(e, "hipMalloc");
+        return p;
+    }
+
+    // Keeps an array that take() gave for `bytes` on device `dev`, if the
+    // cache has room for it; returns whether it did.
+    bool keep(int dev, void* p, uint64_t bytes)
+    {
+        const uint64_t r = rounded(bytes);
+        std::lock_guard<std::mutex> lk(mu);
+        if(kept_bytes[dev] + r > limit(dev))
+            return false;
+        kept.insert({{dev, r}, p});
+        kept_bytes[dev] += r;
+        return true;
+    }
+
+    // Keeps the array, or frees it when the cache is full.
+    void give(int dev, void* p, uint64_t bytes)
+    {
+        if(p != nullptr && !keep(dev, p, bytes))
+            (void)hipFree(p);
+    }
+
+    // Frees the device's cached arrays; returns their bytes.
+    uint64_t trim(int dev)
+    {
+        std::vector<void*> out;
+        uint64_t n = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for(auto it = kept.begin(); it != kept.end();)
+                if(it->first.first == dev)
+                    out.push_back(it->second), n += it->first.second, it = kept.erase(it);
+                else
+                    ++it;
+            kept_bytes[dev] = 0;
+        }
+        for(void* p : out)
+            (void)hipFree(p);
+        return n;
+    }
+
+    uint64_t bytes_kept(int dev)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        return kept_bytes[dev];
+    }
+};
+
+int current_device()
+{
+    int dev = 0;
+    HIP(hipGetDevice(&dev));
+    return dev;
+}
+
 // Host threads that pack sequences into the pinned staging buffers: one
 // packing thread copies at a fraction of the link's speed, so large stages
 // are copied by several.  One job at a time (callers queue on `run_mu`);
@@ -322,6 +438,7 @@ struct kache_table
 
     const double lf;
     const bool map;
+    const int device;                       // the device current at creation: the table's arrays live there
 
     uint64_t cap = 0;                       // buckets
     uint8_t* buckets = nullptr;
@@ -359,7 +476,7 @@ struct kache_table
     mutable size_t ev_used = 0;
 
     kache_table(const kache_options& o)
-        : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0)
+        : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0), device(current_device())
     {
         HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
@@ -400,9 +517,9 @@ struct kache_table
             (void)hipStreamSynchronize(stream);
         free_graphs();
         if(buckets)
-            (void)hipFree(buckets), buckets = nullptr;
+            ArrayCache::get().give(device, buckets, cap * bucket_bytes()), buckets = nullptr;
         if(ov)
-            (void)hipFree(ov), ov = nullptr;
+            ArrayCache::get().give(device, ov, ov_bytes(ov_slots)), ov = nullptr;
         if(d_cnt)
             (void)hipFree(d_cnt), d_cnt = nullptr;
         if(h_cnt)
@@ -436,17 +553,22 @@ struct kache_table
     uint32_t slot_words() const { return map ? 2 : 1; }
     uint64_t vmax() const { return value_bytes == 8 ? ~uint64_t(0) : value_bytes == 0 ? 0 : (uint64_t(1) << (8 * value_bytes)) - 1; }
 
+    uint64_t ov_bytes(uint64_t slots) const { return slots * slot_words() * 8; }
+
+    // Bucket and overflow arrays come from the array cache and go back to it.
+    uint8_t* take_buckets(uint64_t n) const { return static_cast<uint8_t*>(ArrayCache::get().take(device, n * bucket_bytes())); }
+
     uint8_t* alloc_buckets(uint64_t n) const
     {
-        uint8_t* p = dev_alloc<uint8_t>(n * bucket_bytes());
+        uint8_t* p = take_buckets(n);
         zero(p, n * bucket_bytes());
         return p;
     }
 
     uint64_t* alloc_ov(uint64_t slots) const
     {
-        uint64_t* p = dev_alloc<uint64_t>(slots * slot_words());
-        zero(p, slots * slot_words() * 8);
+        uint64_t* p = static_cast<uint64_t*>(ArrayCache::get().take(device, ov_bytes(slots)));
+        zero(p, ov_bytes(slots));
         return p;
     }
 
@@ -578,10 +700,10 @@ struct kache_table
         {
             // The old overflow table is untouched: the table stays as it was.
             (void)hipStreamSynchronize(stream);
-            (void)hipFree(nov);
+            ArrayCache::get().give(device, nov, ov_bytes(new_slots));
             throw;
         }
-        HIP(hipFree(ov));
+        ArrayCache::get().give(device, ov, ov_bytes(ov_slots));
         ov = nov, ov_slots = new_slots;
     }
 
@@ -606,7 +728,7 @@ struct kache_table
             throw std::bad_alloc();
         }
         size_t free_b = 0, total_b = 0;
-        if(hipMemGetInfo(&free_b, &total_b) == hipSuccess && more > free_b)
+        if(hipMemGetInfo(&free_b, &total_b) == hipSuccess && more > free_b + ArrayCache::get().bytes_kept(device))
         {
             last_error = std::string(what) + " needs " + std::to_string(more) + " more bytes of HBM, " + std::to_string(free_b) +
                          " free";
@@ -627,21 +749,25 @@ struct kache_table
 
     // --- Resize: doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681) ---
 
-    // The arrays a doubling replaced are freed by a helper thread, so that
-    // the launches after it need not wait for the unmapping; those frees are
-    // waited for before the next doubling allocates and when the table is
-    // released, so at most one set is pending.
+    // The arrays a doubling replaced go back to the array cache; those it
+    // does not keep are freed by a helper thread, so that the launches after
+    // it need not wait for the unmapping.  Those frees are waited for before
+    // the next doubling allocates and when the table is released, so at most
+    // one set is pending.
     std::vector<std::thread> freeing;
 
-    void free_later(std::vector<void*> ptrs)
+    void free_later(const std::vector<std::pair<void*, uint64_t>>& arrays)
     {
-        int dev = 0;
-        HIP(hipGetDevice(&dev));
-        freeing.emplace_back([dev, ptrs]{
+        std::vector<void*> rest;
+        for(const auto& [p, bytes] : arrays)
+            if(p && !ArrayCache::get().keep(device, p, bytes))
+                rest.push_back(p);
+        if(rest.empty())
+            return;
+        freeing.emplace_back([dev = device, rest]{
             (void)hipSetDevice(dev);
-            for(void* p : ptrs)
-                if(p)
-                    (void)hipFree(p);
+            for(void* p : rest)
+                (void)hipFree(p);
         });
     }
 
@@ -652,12 +778,12 @@ struct kache_table
         freeing.clear();
     }
 
-    // The arrays of the next doubling, allocated ahead by a helper thread
-    // once one is near: mapping gigabytes takes the host a while, which would
+    // The arrays of the next doubling, taken ahead by a helper thread once
+    // one is near: mapping gigabytes takes the host a while, which would
     // otherwise stall the launches around the doubling.
     struct Prepared
     {
-        uint64_t cap = 0, ov_slots = 0, away_bytes = 0;
+        uint64_t cap = 0, ov_slots = 0, bucket_b = 0, ov_b = 0, away_bytes = 0;
         void* buckets = nullptr;
         void* ov = nullptr;
         void* away = nullptr;
@@ -673,27 +799,30 @@ struct kache_table
         wait_frees();
         Prepared p;
         p.cap = 2 * cap, p.ov_slots = next_ov_slots(), p.away_bytes = std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
-        const uint64_t bucket_b = p.cap * bucket_bytes(), ov_b = p.ov_slots * slot_words() * 8;
-        int dev = 0;
-        HIP(hipGetDevice(&dev));
-        prepared = std::async(std::launch::async, [p, bucket_b, ov_b, dev]() mutable
+        p.bucket_b = p.cap * bucket_bytes(), p.ov_b = ov_bytes(p.ov_slots);
+        if(hbm_budget != 0 && device_bytes() + p.bucket_b + p.ov_b + p.away_bytes > hbm_budget)
+            return;     // the doubling would be refused: nothing to prepare
+        prepared = std::async(std::launch::async, [p, dev = device]() mutable
         {
             (void)hipSetDevice(dev);
-            if(hipMalloc(&p.buckets, bucket_b) != hipSuccess)
-                p.buckets = nullptr;
-            if(hipMalloc(&p.ov, ov_b) != hipSuccess)
-                p.ov = nullptr;
-            if(hipMalloc(&p.away, p.away_bytes) != hipSuccess)
-                p.away = nullptr;
+            for(auto [ptr, bytes] : {std::pair<void**, uint64_t>{&p.buckets, p.bucket_b}, {&p.ov, p.ov_b}, {&p.away, p.away_bytes}})
+                try
+                {
+                    *ptr = ArrayCache::get().take(dev, bytes);
+                }
+                catch(...)
+                {
+                    *ptr = nullptr;
+                }
             return p;
         });
     }
 
-    static void free_prepared(const Prepared& p)
+    void free_prepared(const Prepared& p) const
     {
-        for(void* q : {p.buckets, p.ov, p.away})
-            if(q)
-                (void)hipFree(q);
+        ArrayCache::get().give(device, p.buckets, p.bucket_b);
+        ArrayCache::get().give(device, p.ov, p.ov_b);
+        ArrayCache::get().give(device, p.away, p.away_bytes);
     }
 
     // The prepared arrays, if they are those of the doubling about to run.
@@ -721,32 +850,34 @@ struct kache_table
         const uint64_t nov_slots = next_ov_slots();
         const uint64_t pairs = (cap + 1) / 2;
         const Prepared pre = take_prepared(ncap, nov_slots);
+        const uint64_t nb_bytes = ncap * bucket_bytes(), nov_b = ov_bytes(nov_slots);
         uint8_t* nb = nullptr;      // written whole by the split
         uint64_t* nov = nullptr;
-        DevBuf away;                // one bit per old slot: the keys away from their primary bucket
+        // One bit per old slot: the keys away from their primary bucket.
+        uint64_t* away = nullptr;
+        uint64_t away_b = std::max<uint64_t>(pairs * 8, 4096);
         const auto drop_new = [&]
         {
             (void)hipStreamSynchronize(stream);
-            if(nb)
-                (void)hipFree(nb), nb = nullptr;
-            if(nov)
-                (void)hipFree(nov), nov = nullptr;
+            ArrayCache::get().give(device, nb, nb_bytes), nb = nullptr;
+            ArrayCache::get().give(device, nov, nov_b), nov = nullptr;
+            ArrayCache::get().give(device, away, away_b), away = nullptr;
         };
         try
         {
             if(pre.buckets)
             {
                 nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
-                away.p = pre.away, away.bytes = pre.away_bytes;
-                check_budget(ncap * bucket_bytes() + nov_slots * slot_words() * 8 + away.bytes, "doubling the table");
-                zero(nov, nov_slots * slot_words() * 8);
+                away = static_cast<uint64_t*>(pre.away), away_b = pre.away_bytes;
+                check_budget(nb_bytes + nov_b + away_b, "doubling the table");
+                zero(nov, nov_b);
             }
             else
             {
-                check_budget(ncap * bucket_bytes() + nov_slots * slot_words() * 8 + pairs * 8, "doubling the table");
-                nb = dev_alloc<uint8_t>(ncap * bucket_bytes());
+                check_budget(nb_bytes + nov_b + away_b, "doubling the table");
+                nb = take_buckets(ncap);
                 nov = alloc_ov(nov_slots);
-                away.reserve(pairs * 8);
+                away = static_cast<uint64_t*>(ArrayCache::get().take(device, away_b));
             }
         }
         catch(...)
@@ -763,15 +894,15 @@ struct kache_table
             const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
             timed("split", [&]{
                 if(map)
-                    hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
+                    hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away);
                 else
-                    hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away.as<uint64_t>());
+                    hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away);
             });
             timed("reinsert", [&]{
                 if(map)
-                    hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
+                    hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
                 else
-                    hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away.as<uint64_t>(), O, N, d_cnt);
+                    hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
             });
             c = read_counters();
             if(c.failed)
@@ -785,8 +916,7 @@ struct kache_table
             throw;
         }
 
-        free_later({buckets, ov, away.p});
-        away.p = nullptr, away.bytes = 0;
+        free_later({{buckets, cap * bucket_bytes()}, {ov, ov_bytes(ov_slots)}, {away, away_b}});
         buckets = nb, cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = c.ov_inserted;
         resizes++;
         resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
@@ -2046,19 +2176,19 @@ struct kache_table
         wait_frees();
         if(prepared.valid())
             free_prepared(prepared.get());
-        uint8_t* nb = dev_alloc<uint8_t>(h.cap * bucket_bytes());
+        uint8_t* nb = take_buckets(h.cap);
         uint64_t* nov = nullptr;
         try
         {
-            nov = dev_alloc<uint64_t>(h.ov_slots * slot_words());
+            nov = static_cast<uint64_t*>(ArrayCache::get().take(device, ov_bytes(h.ov_slots)));
         }
         catch(...)
         {
-            (void)hipFree(nb);
+            ArrayCache::get().give(device, nb, h.cap * bucket_bytes());
             throw;
         }
-        HIP(hipFree(buckets));
-        HIP(hipFree(ov));
+        ArrayCache::get().give(device, buckets, cap * bucket_bytes());
+        ArrayCache::get().give(device, ov, ov_bytes(ov_slots));
         buckets = nb, ov = nov, cap = h.cap, ov_slots = h.ov_slots;
         HostBuf buf;
         buf.reserve(IO_CHUNK);
@@ -2703,6 +2833,17 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) noexcept
     std::lock_guard<std::mutex> lock(t->mu);
     t->blocks_per_cu = blocks;
     return KACHE_OK;
+}
+
+int kache_amd_trim_cache(uint64_t* released) noexcept
+{
+    return guard([&]()
+    {
+        const uint64_t n = ArrayCache::get().trim(current_device());
+        if(released)
+            *released = n;
+        return static_cast<int>(KACHE_OK);
+    });
 }
 
 int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) noexcept

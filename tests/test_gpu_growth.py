@@ -31,24 +31,26 @@ def same_as_oracle(t, ot, queries):
 
 @pytest.mark.parametrize("width", [0, 4])
 def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
-    g = G.genome(300_000, 91 + width)
-    reads = G.reads(g, 12_000, 150, 92, sub=0.01, nrate=0.001, lower=0.1)
+    # A table of 2M keys (25-45 MB) whose overflow table may grow for a launch
+    # of 4,000 reads but which may not double.
+    g = G.genome(1_500_000, 91 + width)
+    reads = G.reads(g, 48_000, 150, 92, sub=0.01, nrate=0.001, lower=0.1)
     queries = G.reads(g, 500, 150, 93, sub=0.01) + G.pieces(G.genome(5_000, 94), 200)
-    print(f"growth budget, width {width}: {len(reads)} reads in calls of 1000", flush=True)
-    t = capi.Table(api, width, expected_keys=20_000)
+    print(f"growth budget, width {width}: {len(reads)} reads in calls of 4000", flush=True)
+    t = capi.Table(api, width, expected_keys=2_000_000)
     ot = O.OTable(width)
     u = t.register()
     cap0 = t.stats()["capacity"]
-    t.set_hbm_budget(t.info()["device_bytes"] + (64 << 10))     # far below what a doubling needs
+    t.set_hbm_budget(t.info()["device_bytes"] + (32 << 20))    # room for overflow growth, not for a doubling (>= 50 MB)
     refused = None
-    for i in range(0, len(reads), 1000):
-        part = reads[i:i + 1000]
+    for i in range(0, len(reads), 4000):
+        part = reads[i:i + 4000]
         expect = ot.insert(part, capi.ADD, 1)
         try:
             assert t.insert(part, capi.ADD, 1, u) == expect
         except capi.KacheError as e:
             assert e.status == capi.KACHE_ENOMEM
-            assert "budget" in api.last_error()
+            assert "doubling" in api.last_error() and "budget" in api.last_error()
             refused = i
             break
     assert refused is not None, "the table never needed to double"
@@ -57,7 +59,7 @@ def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
     assert t.size() == ot.size()
     same_as_oracle(t, ot, queries)
     # Still refused while the budget holds; the keys of that call are applied too.
-    part = reads[refused + 1000:refused + 2000]
+    part = reads[refused + 4000:refused + 8000]
     ot.insert(part, capi.ADD, 1)
     with pytest.raises(capi.KacheError) as e:
         t.insert(part, capi.ADD, 1, u)
@@ -65,7 +67,7 @@ def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
     same_as_oracle(t, ot, queries)
     # Without the budget the next call doubles, and the table stays exact.
     t.set_hbm_budget(0)
-    part = reads[refused + 2000:refused + 3000]
+    part = reads[refused + 8000:refused + 12000]
     assert t.insert(part, capi.ADD, 1, u) == ot.insert(part, capi.ADD, 1)
     assert t.stats()["capacity"] >= 2 * cap0 and t.info()["resizes"] >= 1
     same_as_oracle(t, ot, queries)
