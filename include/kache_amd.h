@@ -55,6 +55,18 @@ int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const
  * entries, offsets[n] <= text_bytes).  Outputs as the host-sequence calls. */
 int kache_amd_count_seqs_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
                                 uint64_t* counts) KACHE_NOEXCEPT;
+
+/* kache_amd_count_seqs_device without any allocation: no reference
+ * counterpart.  For callers that cannot allocate device or pinned memory
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, checks and outputs; its per-sequence counters
+ * and segment tables live in a fixed per-device scratch array (about 36 MiB,
+ * held until the process ends), so it takes at most 2^20 sequences over at
+ * most 2^30 text bytes (offsets[n]) and returns KACHE_EINVAL beyond that
+ * (kache_amd_last_error says why).  Calls on one device share its scratch and
+ * are serialized by a lock; the scratch keeps the last call's contents. */
+int kache_amd_count_seqs_device_static(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
+                                       uint64_t* counts) KACHE_NOEXCEPT;
 int kache_amd_lookup_groups_device(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
                                    const uint64_t* group_first, size_t n_groups, uint8_t* const* found, void* const* values,
                                    uint8_t* const* fwd) KACHE_NOEXCEPT;

@@ -108,6 +108,7 @@ AMD_SYMBOLS = {
     "kache_amd_host_free": (C.c_int, [_P]),
     "kache_amd_lookup_groups": (C.c_int, [_P, _SEQS, _LENS, C.c_size_t, _P, C.c_size_t, _P, _P, _P]),
     "kache_amd_count_seqs_device": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_size_t, _P]),
+    "kache_amd_count_seqs_device_static": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_size_t, _P]),
     "kache_amd_lookup_groups_device": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P]),
     "kache_amd_insert_text": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kache_amd_update_text": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -750,4 +750,33 @@ __global__ __launch_bounds__(TILE) void k_hashes(const uint64_t* __restrict__ x,
     }
 }
 
+// Segment tables of a per-sequence count over a device text, made on the
+// device: sequence i is segment i (seg_q[i] = i) and its counter starts at 0;
+// tile t's bracket tile_seg[t], for t in [0, tiles], is the last segment that
+// starts at or before position t * TILE (0 if none does), as segment_of reads
+// it.  The host builds the same tables for the per-sequence calls that stage
+// them.
+__global__ __launch_bounds__(TILE) void k_seg_setup(const uint64_t* __restrict__ seg_start, uint64_t ns, uint64_t tiles,
+                                                   uint32_t* __restrict__ seg_q, uint32_t* __restrict__ tile_seg,
+                                                   uint64_t* __restrict__ per_seq)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * TILE;
+    for(uint64_t i = blockIdx.x * uint64_t(TILE) + threadIdx.x; i < ns; i += stride)
+        seg_q[i] = static_cast<uint32_t>(i), per_seq[i] = 0;
+    for(uint64_t t = blockIdx.x * uint64_t(TILE) + threadIdx.x; t <= tiles; t += stride)
+    {
+        const uint64_t p = t * TILE;
+        uint64_t lo = 0, hi = ns - 1;
+        while(lo < hi)
+        {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if(seg_start[mid] <= p)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        tile_seg[t] = static_cast<uint32_t>(lo);
+    }
+}
+
 }

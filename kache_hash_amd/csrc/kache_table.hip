@@ -425,6 +425,62 @@ struct SeqCursor
 
 
 // ---------------------------------------------------------------------------
+// Scratch memory of the allocation-free per-sequence count
+// (kache_amd_count_seqs_device_static): per-sequence counters, segment starts
+// and owners, and tile brackets for at most STATIC_MAX_SEQS sequences over at
+// most STATIC_MAX_TEXT bytes of device text.  One copy per device (about
+// 36 MiB), held until the process ends.  Host code never uses the array's
+// name as an address: each device's copy is found once, at the first table's
+// set-up on that device, by hipGetSymbolAddress, and that address is what the
+// kernels and copies take.
+
+constexpr uint64_t STATIC_MAX_SEQS = uint64_t(1) << 20;
+constexpr uint64_t STATIC_MAX_TEXT = uint64_t(1) << 30;
+constexpr uint64_t SCR_PER_SEQ = 0;
+constexpr uint64_t SCR_SEG_START = SCR_PER_SEQ + STATIC_MAX_SEQS * 8;
+constexpr uint64_t SCR_SEG_Q = SCR_SEG_START + (STATIC_MAX_SEQS + 1) * 8;
+constexpr uint64_t SCR_TILE_SEG = SCR_SEG_Q + STATIC_MAX_SEQS * 4;
+constexpr uint64_t SCRATCH_BYTES = (SCR_TILE_SEG + (STATIC_MAX_TEXT / TILE + 2) * 4 + 255) / 256 * 256;
+
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+
+namespace
+{
+
+struct DeviceScratch
+{
+    uint8_t* ptr = nullptr;     // this device's copy of `scratch`
+    std::mutex calls;           // serializes the calls that use it
+};
+
+std::mutex scratch_mu;
+std::map<int, std::unique_ptr<DeviceScratch>> scratch_of_device;
+
+// The scratch of device `dev`, which must be the current device: its
+// address is resolved on the first call for the device and kept.
+DeviceScratch& device_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    auto& d = scratch_of_device[dev];
+    if(!d)
+    {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || p == nullptr)
+        {
+            (void)hipGetLastError();
+            throw hip_failure(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        auto n = std::make_unique<DeviceScratch>();
+        n->ptr = static_cast<uint8_t*>(p);
+        d = std::move(n);
+    }
+    return *d;
+}
+
+}
+
+// ---------------------------------------------------------------------------
 // The table.
 
 struct kache_table
@@ -439,6 +495,7 @@ struct kache_table
     const double lf;
     const bool map;
     const int device;                       // the device current at creation: the table's arrays live there
+    DeviceScratch* dscratch = nullptr;      // that device's scratch (allocation-free calls)
 
     uint64_t cap = 0;                       // buckets
     uint8_t* buckets = nullptr;
@@ -486,6 +543,7 @@ struct kache_table
             HIP(hipEventCreateWithFlags(&ev_freed[i], hipEventDisableTiming));
         }
 
+        dscratch = &device_scratch(device);
         d_cnt = dev_alloc<Counters>(1);
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(Counters), hipHostMallocDefault));
         zero_counters();
@@ -1945,6 +2003,41 @@ struct kache_table
         HIP(hipGetLastError());
     }
 
+    // per_sequence_device<Q_COUNT> without allocating: the per-sequence
+    // counters and segment tables live in the device's scratch, built there
+    // by k_seg_setup from the offsets; the offsets are copied in and the
+    // counts out by plain copies.  Calls that use one device's scratch are
+    // serialized by its lock.
+    void count_seqs_static(const uint8_t* text, const uint64_t* off, size_t n, uint64_t* out) const
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        std::lock_guard<std::mutex> one(dscratch->calls);
+        std::fill(out, out + n, 0);
+        if(n == 0)
+            return;
+        uint8_t* const scr = dscratch->ptr;
+        uint64_t* const per_seq = reinterpret_cast<uint64_t*>(scr + SCR_PER_SEQ);
+        uint64_t* const seg_start = reinterpret_cast<uint64_t*>(scr + SCR_SEG_START);
+        uint32_t* const seg_q = reinterpret_cast<uint32_t*>(scr + SCR_SEG_Q);
+        uint32_t* const tile_seg = reinterpret_cast<uint32_t*>(scr + SCR_TILE_SEG);
+        const uint64_t text_len = off[n], tiles = (text_len + TILE - 1) / TILE;
+        HIP(hipMemcpyAsync(seg_start, off, (n + 1) * 8, hipMemcpyHostToDevice, stream));
+        const uint64_t setup_blocks = std::min<uint64_t>((std::max<uint64_t>(n, tiles + 1) + TILE - 1) / TILE, 4096);
+        hipLaunchKernelGGL(k_seg_setup, dim3(setup_blocks), dim3(TILE), 0, stream, seg_start, uint64_t(n), tiles, seg_q, tile_seg, per_seq);
+        HIP(hipGetLastError());
+        QueryArgs a{};
+        a.seg_start = seg_start, a.seg_q = seg_q, a.tile_seg = tile_seg, a.per_seq = per_seq;
+        if(map)
+            hipLaunchKernelGGL((k_query<true, Q_COUNT>), dim3(grid_for(k_query<true, Q_COUNT>, tiles)), dim3(TILE), 0, stream, view(), text,
+                               uint64_t(0), text_len, a, d_cnt);
+        else
+            hipLaunchKernelGGL((k_query<false, Q_COUNT>), dim3(grid_for(k_query<false, Q_COUNT>, tiles)), dim3(TILE), 0, stream, view(), text,
+                               uint64_t(0), text_len, a, d_cnt);
+        HIP(hipGetLastError());
+        HIP(hipMemcpyAsync(out, per_seq, n * 8, hipMemcpyDeviceToHost, stream));
+        HIP(hipStreamSynchronize(stream));
+    }
+
     Counters count_device(const uint8_t* text, uint64_t text_len) const
     {
         std::lock_guard<std::mutex> lock(mu);
@@ -2432,6 +2525,25 @@ int kache_amd_count_seqs_device(const kache_table* t, const char* text, uint64_t
     return guard([&]()
     {
         t->per_sequence_device<Q_COUNT>(reinterpret_cast<const uint8_t*>(text), offsets, n, counts, 0, 0);
+        return static_cast<int>(KACHE_OK);
+    });
+}
+
+int kache_amd_count_seqs_device_static(const kache_table* t, const char* text, uint64_t text_bytes, const uint64_t* offsets, size_t n,
+                                       uint64_t* counts) noexcept
+{
+    if(t == nullptr || !offsets_ok(text, text_bytes, offsets, n) || (n > 0 && counts == nullptr))
+        return KACHE_EINVAL;
+    if(n > STATIC_MAX_SEQS || (n > 0 && offsets[n] > STATIC_MAX_TEXT))
+    {
+        last_error = "kache_amd_count_seqs_device_static: " + std::to_string(n) + " sequences over " +
+                     std::to_string(n ? offsets[n] : 0) + " text bytes exceed its scratch (at most " +
+                     std::to_string(STATIC_MAX_SEQS) + " sequences and " + std::to_string(STATIC_MAX_TEXT) + " bytes)";
+        return KACHE_EINVAL;
+    }
+    return guard([&]()
+    {
+        t->count_seqs_static(reinterpret_cast<const uint8_t*>(text), offsets, n, counts);
         return static_cast<int>(KACHE_OK);
     });
 }
