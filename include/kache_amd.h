@@ -215,6 +215,9 @@ typedef struct kache_amd_info
     uint64_t resizes;
     uint64_t device_bytes;      /* bucket store + overflow table */
     uint64_t value_bytes;       /* 0 for a set */
+    uint64_t next_doubling_bytes;   /* HBM the next doubling needs beside the table's arrays */
+    uint64_t device_free_bytes;     /* free HBM of the table's device, arrays kept for reuse included */
+    uint64_t hbm_budget;            /* kache_amd_set_hbm_budget (0: none) */
 } kache_amd_info;
 int kache_amd_get_info(const kache_table* t, kache_amd_info* out) KACHE_NOEXCEPT;
 
@@ -256,13 +259,19 @@ const char* kache_amd_synth_error(void) KACHE_NOEXCEPT;
  * waves: a stager packs wave k + 1's sequences into one of two pinned
  * buffers and copies them to the device (its HBM budget: two such texts)
  * while an executor runs wave k from the other; a wave is admitted when a
- * buffer is free, and requests arriving meanwhile join it.  A wave runs in
- * three phases: its reads (COUNT and LOOKUP, each kind merged into one
- * device call), its updates, then its upserts (one pipelined batch call per
- * run of equal op and operand, one batch per request); a request completes
- * when its phase ends.  Each request's results are those of running all
- * requests alone, one after another, in served order: by wave, then phase
- * (reads 0, updates 1, upserts 2), then ticket.  With max_wave_requests = 1
+ * buffer is free, and requests arriving meanwhile join it.  A wave runs as
+ * device calls one after another, each over requests of one kind (per-
+ * sequence counts, lookups, one update, or upserts of one op and operand,
+ * one batch per request), its requests in ticket order; a request completes
+ * when its call ends.  Unless more requests were waiting when the wave was
+ * taken, its calls run shortest first across kinds (requests in size order,
+ * consecutive ones of one kind merged while none is larger than the call so
+ * far), so that a small request never waits for a large one of its wave;
+ * a wave taken from a backlog runs its counts in one call, its lookups in
+ * one, its updates, then its upserts in ticket order by runs of equal op and
+ * operand.  Each request's results are those of running all requests alone,
+ * one after another, in served order: by wave, then call (`phase`), then
+ * ticket.  With max_wave_requests = 1
  * there is no pipeline: every request is one plain C-ABI call on one
  * worker thread (the per-request baseline).
  *
@@ -292,7 +301,7 @@ typedef struct kache_amd_request
 typedef struct kache_amd_request_result
 {
     int status;                 /* the call's status */
-    uint32_t phase;             /* 0 reads, 1 updates, 2 upserts */
+    uint32_t phase;             /* its call's index within the wave (calls run one after another) */
     uint64_t ticket;
     uint64_t wave;              /* 1, 2, ... */
     double submit_s;            /* kache_amd_clock() at submission, start of its wave, completion */
@@ -307,6 +316,7 @@ typedef struct kache_amd_server_stats
     uint64_t max_wave_requests;
     uint64_t max_wave_bytes;
     uint64_t direct_waves;      /* lone requests run at an idle server by the plain calls, unstaged */
+    uint64_t refused;           /* requests completed with KACHE_ENOMEM by admission (HBM budget) */
 } kache_amd_server_stats;
 
 typedef struct kache_amd_server kache_amd_server;
@@ -318,6 +328,15 @@ int kache_amd_server_submit(kache_amd_server* s, const kache_amd_request* r, uin
 /* Blocks until the request is done; each ticket is waited for once. */
 int kache_amd_server_wait(kache_amd_server* s, uint64_t ticket, kache_amd_request_result* res) KACHE_NOEXCEPT;
 int kache_amd_server_get_stats(kache_amd_server* s, kache_amd_server_stats* out) KACHE_NOEXCEPT;
+/* Admission by HBM budget.  A request whose staged sequences exceed its
+ * wave slot runs alone in a slot grown for it, admitted only if the growth
+ * fits both the server's staging budget (the bytes of its slots together;
+ * 0, the default, sets none) and the device's free memory less what the
+ * table's next doubling needs (kache_amd_info.next_doubling_bytes), so that
+ * staging never takes the memory the table needs to grow; otherwise the
+ * request completes with KACHE_ENOMEM without running, and the server goes
+ * on with the next (kache_amd_server_stats.refused). */
+int kache_amd_server_set_hbm_budget(kache_amd_server* s, uint64_t bytes) KACHE_NOEXCEPT;
 /* Open-loop load: submits request i at `arrival_s[i]` seconds (non-decreasing)
  * after the call starts, from the calling thread, while another thread waits
  * for them in order; returns when all are done, with results[i] for each. */

@@ -35,7 +35,8 @@ class Stats(C.Structure):
 class Info(C.Structure):
     _fields_ = [("buckets", C.c_uint64), ("bucket_bytes", C.c_uint64), ("overflow_slots", C.c_uint64),
                 ("overflow_keys", C.c_uint64), ("resizes", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("value_bytes", C.c_uint64)]
+                ("value_bytes", C.c_uint64), ("next_doubling_bytes", C.c_uint64), ("device_free_bytes", C.c_uint64),
+                ("hbm_budget", C.c_uint64)]
 
 
 REQ_COUNT, REQ_LOOKUP, REQ_UPDATE, REQ_INSERT = range(4)   # KACHE_AMD_REQ_*
@@ -61,7 +62,7 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("waves", C.c_uint64), ("requests", C.c_uint64), ("max_wave_requests", C.c_uint64),
-                ("max_wave_bytes", C.c_uint64), ("direct_waves", C.c_uint64)]
+                ("max_wave_bytes", C.c_uint64), ("direct_waves", C.c_uint64), ("refused", C.c_uint64)]
 
 
 KEYS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_size_t)
@@ -143,6 +144,7 @@ AMD_SYMBOLS = {
     "kache_amd_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     "kache_amd_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestResult)]),
     "kache_amd_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "kache_amd_server_set_hbm_budget": (C.c_int, [_P, C.c_uint64]),
     "kache_amd_server_replay": (C.c_int, [_P, C.POINTER(Request), _P, C.c_size_t, C.POINTER(RequestResult)]),
     "kache_amd_clock": (C.c_double, []),
     "kache_amd_host_minimizer_hash": (C.c_uint64, [C.c_uint64]),
@@ -584,6 +586,10 @@ class Server:
         out = (RequestResult * max(n, 1))()
         self.api.check(self.api.lib.kache_amd_server_replay(self.h, arr, times.ctypes.data, n, out), "kache_amd_server_replay")
         return [{f: getattr(out[i], f) for f, _ in RequestResult._fields_} for i in range(n)]
+
+    def set_hbm_budget(self, nbytes):
+        """Staging bytes the server's slots may hold together (0: no budget of its own)."""
+        self.api.check(self.api.lib.kache_amd_server_set_hbm_budget(self.h, nbytes), "kache_amd_server_set_hbm_budget")
 
     def stats(self):
         s = ServerStats()

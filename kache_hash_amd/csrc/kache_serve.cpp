@@ -97,32 +97,41 @@ uint32_t phase_of(uint32_t kind)
 
 }
 
-// A wave's reads of one kind (counts or lookups) run in chunks of ascending
-// size, one device call each, whose requests complete together: a chunk ends
-// where the next request is larger than the whole chunk, so that no request
-// waits for one much larger than itself, in few calls.  Reads commute, so
-// their order within the phase changes no result.  A wave taken while more
-// requests wait (the server is saturated, and throughput, not the latency of
-// one wave, decides) runs each kind in one call.
-struct ReadChunk
+// A staged wave runs as a list of device calls, one after another; the
+// requests of one call run in ticket order within it, so each request's
+// results are those of a serial run in served order (wave, call, ticket).
+// A call holds requests of one kind: per-sequence counts, lookups (one
+// output group per request), one present-only update, or upserts of one
+// operation and operand (one batch per request, so each request's count of
+// new keys is its own).
+//
+// Unless the server is saturated, the calls run shortest first across kinds:
+// the wave's requests in order of size, consecutive ones of one kind (and for
+// upserts one operation) merged into a call while no request is larger than
+// the call so far.  Any serial order of the requests of one wave is a valid
+// one (they were all pending together), and this one keeps a small request
+// from waiting for a large one of another kind.  A wave taken while more
+// requests wait (saturated: throughput, not one wave's latency, decides) runs
+// its counts in one call, its lookups in one, each update, then its upserts
+// in ticket order, one call per run of equal operation and operand.
+struct Call
 {
-    size_t begin = 0, end = 0;          // requests [begin, end) of the kind, in size order
-    uint64_t at = 0;                    // start in the slot's text (16-byte aligned)
-    std::vector<uint64_t> off;          // sequence offsets from `at`, and the end
+    uint32_t kind = 0, op = 0;
+    uint64_t x = 0;
+    std::vector<Req*> reqs;             // in ticket order
+    uint64_t bytes = 0;                 // staged bytes of its requests
+    uint64_t at = 0;                    // start of its region in the slot's text (16-byte aligned)
+    std::vector<uint64_t> off;          // reads: sequence offsets from `at`, and the end; upserts: batch bounds
 };
 
-// A wave staged for the device: its requests by phase, and where their
+// A wave staged for the device: its requests, its calls and where their
 // sequences sit in the slot's text (each sequence followed by '\n').
 struct Staged
 {
     uint64_t id = 0;
     int slot = 0;
     std::vector<std::shared_ptr<Req>> reqs;
-    std::vector<Req*> counts, lookups, updates, inserts;
-    std::vector<ReadChunk> count_chunks, lookup_chunks;
-    uint64_t insert_at = 0;                                    // region start (16-byte aligned)
-    std::vector<uint64_t> update_at, update_bytes;             // per update request
-    std::vector<uint64_t> insert_bounds;                       // per insert request, within its region
+    std::vector<Call> calls;
     uint64_t bytes = 0;                                        // staged text bytes
     bool direct = false;                                       // not staged: run by the plain calls
     bool saturated = false;                                    // more requests waited when it was taken
@@ -162,6 +171,7 @@ struct kache_amd_server
     bool exec_busy = false;     // the executor is running a wave
 
     kache_amd_server_stats stats{};
+    std::atomic<uint64_t> hbm_budget{0};       // staging bytes of the slots together (0: no budget of its own)
 
     kache_amd_server(kache_table* t, uint32_t user, uint64_t wave_bytes, uint32_t max_req, uint32_t width)
         : t(t), user(user), wave_bytes(wave_bytes), max_wave_requests(max_req), width(width)
@@ -458,57 +468,88 @@ struct kache_amd_server
 
     static uint64_t align16(uint64_t x) { return (x + 15) & ~uint64_t(15); }
 
-    // Packs a wave into slot `w.slot`'s pinned text and copies it to the device.
+    // Admission of `extra` more bytes of staging for a request larger than
+    // its slot: within the server's budget, if it has one, and within the
+    // device's free memory less what the table's next doubling needs.
+    int admit_growth(uint64_t extra)
+    {
+        kache_amd_info info{};
+        int st = kache_amd_get_info(t, &info);
+        const uint64_t budget = hbm_budget.load();
+        if(st == KACHE_OK && budget != 0 && slot_bytes[0] + slot_bytes[1] + extra > budget)
+            st = KACHE_ENOMEM;
+        if(st == KACHE_OK && info.device_free_bytes < info.next_doubling_bytes + extra)
+            st = KACHE_ENOMEM;
+        if(st == KACHE_ENOMEM)
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stats.refused++;
+        }
+        return st;
+    }
+
+    // The calls of a wave, in run order (see Call).
+    static void plan(Staged& w)
+    {
+        const auto same_call = [](const Call& c, const Req* q)
+        {
+            return c.kind == q->r.kind && c.kind != KACHE_AMD_REQ_UPDATE &&
+                   (c.kind != KACHE_AMD_REQ_INSERT || (c.op == q->r.op && c.x == q->r.x));
+        };
+        const auto open = [&](const Req* q)
+        {
+            Call c;
+            c.kind = q->r.kind, c.op = q->r.op, c.x = q->r.x;
+            w.calls.push_back(std::move(c));
+        };
+        std::vector<Req*> rs;
+        for(auto& q : w.reqs)
+            rs.push_back(q.get());
+        if(w.saturated)
+        {
+            // Ticket order already; reads by kind, then updates, then upserts.
+            for(uint32_t kind : {KACHE_AMD_REQ_COUNT, KACHE_AMD_REQ_LOOKUP, KACHE_AMD_REQ_UPDATE, KACHE_AMD_REQ_INSERT})
+            {
+                const size_t first = w.calls.size();
+                for(Req* q : rs)
+                    if(q->r.kind == kind)
+                    {
+                        if(w.calls.size() == first || !same_call(w.calls.back(), q))
+                            open(q);
+                        w.calls.back().reqs.push_back(q), w.calls.back().bytes += q->bytes;
+                    }
+            }
+            return;
+        }
+        std::sort(rs.begin(), rs.end(), [](const Req* a, const Req* b) { return a->bytes < b->bytes || (a->bytes == b->bytes && a->ticket < b->ticket); });
+        for(Req* q : rs)
+        {
+            if(w.calls.empty() || !same_call(w.calls.back(), q) || q->bytes > w.calls.back().bytes)
+                open(q);
+            w.calls.back().reqs.push_back(q), w.calls.back().bytes += q->bytes;
+        }
+        for(Call& c : w.calls)
+            std::sort(c.reqs.begin(), c.reqs.end(), [](const Req* a, const Req* b) { return a->ticket < b->ticket; });
+    }
+
+    // Plans a wave, packs it into slot `w.slot`'s pinned text (one 16-byte
+    // aligned region per call) and copies it to the device.
     int pack(Staged& w)
     {
-        for(auto& q : w.reqs)
-            switch(q->r.kind)
-            {
-            case KACHE_AMD_REQ_COUNT: w.counts.push_back(q.get()); break;
-            case KACHE_AMD_REQ_LOOKUP: w.lookups.push_back(q.get()); break;
-            case KACHE_AMD_REQ_UPDATE: w.updates.push_back(q.get()); break;
-            default: w.inserts.push_back(q.get()); break;
-            }
-        // Layout: count chunks, lookup chunks, each update request, inserts;
-        // every region 16-byte aligned.
-        const auto by_size = [](const Req* a, const Req* b) { return a->bytes < b->bytes || (a->bytes == b->bytes && a->ticket < b->ticket); };
-        std::sort(w.counts.begin(), w.counts.end(), by_size);
-        std::sort(w.lookups.begin(), w.lookups.end(), by_size);
+        plan(w);
         uint64_t o = 0;
-        const auto chunks = [&](const std::vector<Req*>& rs, std::vector<ReadChunk>& out)
-        {
-            for(size_t i = 0; i < rs.size();)
-            {
-                ReadChunk c;
-                c.begin = i, c.at = o;
-                uint64_t b = 0;
-                do
-                    b += rs[i++]->bytes;
-                while(i < rs.size() && (w.saturated || rs[i]->bytes <= b));
-                c.end = i;
-                o = align16(o + b);
-                out.push_back(std::move(c));
-            }
-        };
-        const auto region = [&](const std::vector<Req*>& rs)
-        {
-            uint64_t b = 0;
-            for(Req* q : rs)
-                b += q->bytes;
-            const uint64_t at = o;
-            o = align16(o + b);
-            return at;
-        };
-        chunks(w.counts, w.count_chunks);
-        chunks(w.lookups, w.lookup_chunks);
-        for(Req* q : w.updates)
-            w.update_at.push_back(o), w.update_bytes.push_back(q->bytes), o = align16(o + q->bytes);
-        w.insert_at = region(w.inserts);
+        for(Call& c : w.calls)
+            c.at = o, o = align16(o + c.bytes);
         w.bytes = o;
         const uint64_t need = o + KACHE_AMD_TEXT_PAD;
         const int sl = w.slot;
         if(need > slot_bytes[sl])
         {
+            // A request larger than the slot: it runs alone, in a slot grown
+            // for it, if the HBM it needs is free (admission by budget).
+            const int adm = admit_growth(need - slot_bytes[sl]);
+            if(adm != KACHE_OK)
+                return adm;
             if(h_slot[sl])
                 kache_amd_host_free(h_slot[sl]), h_slot[sl] = nullptr;
             if(d_slot[sl])
@@ -533,42 +574,29 @@ struct kache_amd_server
         std::vector<const char*> src;
         std::vector<uint64_t> len;
         uint64_t filled = 0;
-        const auto put = [&](const kache_amd_request& r, uint64_t at, std::vector<uint64_t>* offs)
+        for(Call& c : w.calls)
         {
+            uint64_t at = c.at;
             if(at > filled)
                 std::memset(h + filled, '\n', at - filled);
-            for(size_t j = 0; j < r.n; ++j)
+            const bool reads = c.kind == KACHE_AMD_REQ_COUNT || c.kind == KACHE_AMD_REQ_LOOKUP;
+            if(c.kind == KACHE_AMD_REQ_INSERT)
+                c.off.push_back(0);
+            for(Req* q : c.reqs)
             {
-                if(offs)
-                    offs->push_back(at);
-                dst.push_back(h + at), src.push_back(r.seqs[j]), len.push_back(r.lens[j]);
-                at += r.lens[j] + 1;
+                for(size_t j = 0; j < q->r.n; ++j)
+                {
+                    if(reads)
+                        c.off.push_back(at - c.at);
+                    dst.push_back(h + at), src.push_back(q->r.seqs[j]), len.push_back(q->r.lens[j]);
+                    at += q->r.lens[j] + 1;
+                }
+                if(c.kind == KACHE_AMD_REQ_INSERT)
+                    c.off.push_back(at - c.at);
             }
+            if(reads)
+                c.off.push_back(at - c.at);
             filled = at;
-            return at;
-        };
-        const auto fill_chunks = [&](const std::vector<Req*>& rs, std::vector<ReadChunk>& cs)
-        {
-            for(ReadChunk& c : cs)
-            {
-                uint64_t a = c.at;
-                for(size_t k = c.begin; k < c.end; ++k)
-                    a = put(rs[k]->r, a, &c.off);
-                c.off.push_back(a);
-                for(auto& x : c.off)
-                    x -= c.at;
-            }
-        };
-        fill_chunks(w.counts, w.count_chunks);
-        fill_chunks(w.lookups, w.lookup_chunks);
-        for(size_t i = 0; i < w.updates.size(); ++i)
-            put(w.updates[i]->r, w.update_at[i], nullptr);
-        uint64_t at = w.insert_at;
-        w.insert_bounds.push_back(0);
-        for(Req* q : w.inserts)
-        {
-            at = put(q->r, at, nullptr);
-            w.insert_bounds.push_back(at - w.insert_at);
         }
         if(need > filled)
             std::memset(h + filled, '\n', need - filled);
@@ -592,8 +620,10 @@ struct kache_amd_server
                 }
                 // A lone request reaching an idle server is not staged: the
                 // executor runs it at once by the plain calls (whose own
-                // pipeline copies it), one copy and one hand-off fewer.
-                if(pending.size() == 1 && staged.empty() && !exec_busy)
+                // pipeline copies it), one copy and one hand-off fewer.  One
+                // larger than a slot is staged, through admission.
+                if(pending.size() == 1 && staged.empty() && !exec_busy &&
+                   pending.front()->bytes + KACHE_AMD_TEXT_PAD <= std::min(slot_bytes[0], slot_bytes[1]))
                 {
                     w->direct = true;
                     w->reqs = take_wave(w->id);
@@ -681,71 +711,60 @@ struct kache_amd_server
     void execute(Staged& w)
     {
         const double t0 = now_s();
-        for(auto& q : w.reqs)
-            q->t_start = t0, q->wave = w.id, q->phase = phase_of(q->r.kind);
+        for(size_t k = 0; k < w.calls.size(); ++k)
+            for(Req* q : w.calls[k].reqs)
+                q->t_start = t0, q->wave = w.id, q->phase = static_cast<uint32_t>(k);
         const char* d = static_cast<const char*>(d_slot[w.slot]);
-
-        for(const ReadChunk& c : w.count_chunks)
+        for(const Call& c : w.calls)
         {
-            const size_t n = c.off.size() - 1;
-            std::vector<uint64_t> out(std::max<size_t>(n, 1));
-            const int st = kache_amd_count_seqs_device(t, d + c.at, c.off.back(), c.off.data(), n, out.data());
-            std::vector<Req*> done(w.counts.begin() + c.begin, w.counts.begin() + c.end);
-            size_t o = 0;
-            for(Req* q : done)
+            const char* text = d + c.at;
+            if(c.kind == KACHE_AMD_REQ_COUNT)
             {
-                q->status = st;
-                if(st == KACHE_OK && q->r.n)
-                    std::memcpy(q->r.counts, out.data() + o, q->r.n * sizeof(uint64_t));
-                o += q->r.n;
+                const size_t n = c.off.size() - 1;
+                std::vector<uint64_t> out(std::max<size_t>(n, 1));
+                const int st = kache_amd_count_seqs_device(t, text, c.off.back(), c.off.data(), n, out.data());
+                size_t o = 0;
+                for(Req* q : c.reqs)
+                {
+                    q->status = st;
+                    if(st == KACHE_OK && q->r.n)
+                        std::memcpy(q->r.counts, out.data() + o, q->r.n * sizeof(uint64_t));
+                    o += q->r.n;
+                }
             }
-            finish(done);
-        }
-        for(const ReadChunk& c : w.lookup_chunks)
-        {
-            std::vector<Req*> done(w.lookups.begin() + c.begin, w.lookups.begin() + c.end);
-            std::vector<uint64_t> first;
-            std::vector<uint8_t*> found, fwd;
-            std::vector<void*> values;
-            uint64_t o = 0;
-            for(Req* q : done)
+            else if(c.kind == KACHE_AMD_REQ_LOOKUP)
             {
-                first.push_back(o), o += q->r.n;
-                found.push_back(q->r.found), values.push_back(q->r.values), fwd.push_back(q->r.fwd);
+                std::vector<uint64_t> first;
+                std::vector<uint8_t*> found, fwd;
+                std::vector<void*> values;
+                uint64_t o = 0;
+                for(Req* q : c.reqs)
+                {
+                    first.push_back(o), o += q->r.n;
+                    found.push_back(q->r.found), values.push_back(q->r.values), fwd.push_back(q->r.fwd);
+                }
+                const int st = kache_amd_lookup_groups_device(t, text, c.off.back(), c.off.data(), c.off.size() - 1, first.data(), c.reqs.size(),
+                                                              found.data(), values.data(), fwd.data());
+                for(Req* q : c.reqs)
+                    q->status = st;
             }
-            const int st = kache_amd_lookup_groups_device(t, d + c.at, c.off.back(), c.off.data(), c.off.size() - 1, first.data(), done.size(),
-                                                          found.data(), values.data(), fwd.data());
-            for(Req* q : done)
-                q->status = st;
-            finish(done);
-        }
-        for(size_t i = 0; i < w.updates.size(); ++i)
-        {
-            Req* q = w.updates[i];
-            q->status = kache_amd_update_device(t, d + w.update_at[i], w.update_bytes[i], q->r.op, q->r.x, q->r.counts);
-        }
-        finish(w.updates);
-        // Upserts in ticket order, one pipelined call per run of equal op and
-        // operand; unless the wave is saturated, a call also ends before a
-        // request larger than the call so far, and its requests complete then.
-        for(size_t a = 0; a < w.inserts.size();)
-        {
-            size_t b = a + 1;
-            uint64_t bytes = w.inserts[a]->bytes;
-            while(b < w.inserts.size() && w.inserts[b]->r.op == w.inserts[a]->r.op && w.inserts[b]->r.x == w.inserts[a]->r.x &&
-                  (w.saturated || w.inserts[b]->bytes <= bytes))
-                bytes += w.inserts[b++]->bytes;
-            std::vector<uint64_t> out(b - a);
-            const int st = kache_amd_insert_batches(t, user, d + w.insert_at, w.insert_bounds.data() + a, b - a, w.inserts[a]->r.op,
-                                                    w.inserts[a]->r.x, out.data());
-            for(size_t i = a; i < b; ++i)
+            else if(c.kind == KACHE_AMD_REQ_UPDATE)
             {
-                w.inserts[i]->status = st;
-                if(st == KACHE_OK && w.inserts[i]->r.counts)
-                    *w.inserts[i]->r.counts = out[i - a];
+                Req* q = c.reqs[0];
+                q->status = kache_amd_update_device(t, text, c.bytes, q->r.op, q->r.x, q->r.counts);
             }
-            finish(std::vector<Req*>(w.inserts.begin() + a, w.inserts.begin() + b));
-            a = b;
+            else
+            {
+                std::vector<uint64_t> out(c.reqs.size());
+                const int st = kache_amd_insert_batches(t, user, text, c.off.data(), c.reqs.size(), c.op, c.x, out.data());
+                for(size_t i = 0; i < c.reqs.size(); ++i)
+                {
+                    c.reqs[i]->status = st;
+                    if(st == KACHE_OK && c.reqs[i]->r.counts)
+                        *c.reqs[i]->r.counts = out[i];
+                }
+            }
+            finish(c.reqs);
         }
     }
 };
@@ -836,6 +855,14 @@ int kache_amd_server_wait(kache_amd_server* s, uint64_t ticket, kache_amd_reques
         res->start_s = q->t_start;
         res->done_s = q->t_done;
     }
+    return KACHE_OK;
+}
+
+int kache_amd_server_set_hbm_budget(kache_amd_server* s, uint64_t bytes) noexcept
+{
+    if(s == nullptr)
+        return KACHE_EINVAL;
+    s->hbm_budget.store(bytes);
     return KACHE_OK;
 }
 

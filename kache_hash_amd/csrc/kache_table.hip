@@ -519,6 +519,9 @@ struct kache_table
     mutable DevBuf d_pipe[2];
     mutable hipStream_t copy_stream = nullptr;
     mutable hipEvent_t ev_copied[2] = {}, ev_freed[2] = {};
+    // Double-buffered download of large outputs to the caller's memory.
+    mutable HostBuf h_dl[2];
+    mutable hipEvent_t ev_dl[2] = {};
     mutable DevBuf d_text, d_misc, d_out, d_out2, d_out3;
     mutable Counters* d_cnt = nullptr;
     mutable Counters* h_cnt = nullptr;
@@ -541,6 +544,7 @@ struct kache_table
         {
             HIP(hipEventCreateWithFlags(&ev_copied[i], hipEventDisableTiming));
             HIP(hipEventCreateWithFlags(&ev_freed[i], hipEventDisableTiming));
+            HIP(hipEventCreateWithFlags(&ev_dl[i], hipEventDisableTiming));
         }
 
         dscratch = &device_scratch(device);
@@ -602,6 +606,8 @@ struct kache_table
                 (void)hipEventDestroy(ev_copied[i]), ev_copied[i] = nullptr;
             if(ev_freed[i])
                 (void)hipEventDestroy(ev_freed[i]), ev_freed[i] = nullptr;
+            if(ev_dl[i])
+                (void)hipEventDestroy(ev_dl[i]), ev_dl[i] = nullptr;
         }
         if(stream)
             (void)hipStreamDestroy(stream), stream = nullptr;
@@ -772,6 +778,13 @@ struct kache_table
     uint64_t hbm_budget = 0;
 
     uint64_t device_bytes() const { return cap * bucket_bytes() + ov_slots * slot_words() * 8; }
+
+    // HBM the next doubling needs beside the current arrays: the doubled
+    // bucket store, its overflow table and the split's bitmap.
+    uint64_t next_doubling_bytes() const
+    {
+        return 2 * cap * bucket_bytes() + ov_bytes(next_ov_slots()) + std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
+    }
 
     // Throws bad_alloc (KACHE_ENOMEM) before growth that would need `more`
     // bytes beside the table's current arrays, beyond the budget or beyond
@@ -1930,6 +1943,40 @@ struct kache_table
     // a lookup of its own sequences alone, are copied to its destinations.
     // With `dev_text`, the sequences are already there, at offsets `dev_off`
     // (as per_sequence_device takes them), and seqs / lens are unused.
+    // Copies `n` bytes of device memory to the caller's memory, in stream
+    // order: a small copy is only enqueued (the caller synchronizes the
+    // stream before it reads); a large one goes through two pinned chunks,
+    // the copy engine filling chunk i + 1 while the packing threads copy
+    // chunk i out, several times the rate of a copy into pageable memory,
+    // and has ended when this returns.
+    void download(void* dst, const void* src, uint64_t n) const
+    {
+        constexpr uint64_t CHUNK = uint64_t(8) << 20;
+        if(n < 2 * CHUNK)
+        {
+            HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
+            return;
+        }
+        h_dl[0].reserve(CHUNK), h_dl[1].reserve(CHUNK);
+        const uint64_t chunks = (n + CHUNK - 1) / CHUNK;
+        const auto issue = [&](uint64_t i)
+        {
+            const uint64_t a = i * CHUNK, len = std::min(CHUNK, n - a);
+            HIP(hipMemcpyAsync(h_dl[i % 2].p, static_cast<const char*>(src) + a, len, hipMemcpyDeviceToHost, stream));
+            HIP(hipEventRecord(ev_dl[i % 2], stream));
+        };
+        issue(0);
+        for(uint64_t i = 0; i < chunks; ++i)
+        {
+            // Chunk i + 1's buffer held chunk i - 1, copied out in the last turn.
+            if(i + 1 < chunks)
+                issue(i + 1);
+            HIP(hipEventSynchronize(ev_dl[i % 2]));
+            const uint64_t a = i * CHUNK;
+            par_copy(static_cast<char*>(dst) + a, h_dl[i % 2].as<char>(), std::min(CHUNK, n - a));
+        }
+    }
+
     void lookup_groups(const char* const* seqs, const size_t* lens, size_t n, const LookupDest* g, size_t ng,
                        const uint8_t* dev_text = nullptr, const uint64_t* dev_off = nullptr) const
     {
@@ -1987,11 +2034,11 @@ struct kache_table
             const uint64_t b = o[g[i].first], e = o[i + 1 < ng ? g[i + 1].first : n];
             if(e == b)
                 continue;
-            HIP(hipMemcpyAsync(g[i].found, d_out.as<uint8_t>() + b, e - b, hipMemcpyDeviceToHost, stream));
+            download(g[i].found, d_out.as<uint8_t>() + b, e - b);
             if(g[i].values)
-                HIP(hipMemcpyAsync(g[i].values, d_out2.as<uint8_t>() + b * vb, (e - b) * vb, hipMemcpyDeviceToHost, stream));
+                download(g[i].values, d_out2.as<uint8_t>() + b * vb, (e - b) * vb);
             if(g[i].fwd)
-                HIP(hipMemcpyAsync(g[i].fwd, d_out3.as<uint8_t>() + b, e - b, hipMemcpyDeviceToHost, stream));
+                download(g[i].fwd, d_out3.as<uint8_t>() + b, e - b);
         }
         sync();
     }
@@ -3019,8 +3066,12 @@ int kache_amd_get_info(const kache_table* t, kache_amd_info* out) noexcept
     out->overflow_slots = t->ov_slots;
     out->overflow_keys = t->ov_count;
     out->resizes = t->resizes;
-    out->device_bytes = t->cap * t->bucket_bytes() + t->ov_slots * t->slot_words() * 8;
+    out->device_bytes = t->device_bytes();
     out->value_bytes = t->value_bytes;
+    out->next_doubling_bytes = t->next_doubling_bytes();
+    size_t free_b = 0, total_b = 0;
+    out->device_free_bytes = (hipMemGetInfo(&free_b, &total_b) == hipSuccess ? free_b : 0) + ArrayCache::get().bytes_kept(t->device);
+    out->hbm_budget = t->hbm_budget;
     return KACHE_OK;
 }
 

@@ -307,3 +307,47 @@ def test_device_text_queries_equal_host_queries(api):
     bad = off.copy()
     bad[3] = bad[2]                       # an empty span: no room for the separator
     assert api.lib.kache_amd_count_seqs_device(t.h, d.ptr, len(text), bad.ctypes.data, len(seqs), out.ctypes.data) == capi.KACHE_EINVAL
+
+
+def test_serve_hbm_budget_refuses_an_oversized_request(api):
+    """Admission by HBM budget: with the server's staging budget at its two
+    slots, a request larger than a slot completes with KACHE_ENOMEM without
+    running, whether it arrives alone or among others; every other request
+    runs, with the outputs of a serial run in served order."""
+    g = G.genome(200_000, 71)
+    rng = np.random.default_rng(71)
+    prefill = random_seqs(rng, g, 300)
+    wave = 64 << 10
+    t = capi.Table(api, 4, expected_keys=100_000)
+    u = t.register()
+    t.insert(prefill, capi.ADD, 1, u)
+    o = O.OTable(4)
+    o.insert(prefill, capi.ADD, 1)
+    big = [g[i:i + 2000] for i in range(0, 200_000 - 2000, 1500)]       # ~ 265 KB staged: over the slot
+    smalls = [random_seqs(rng, g, 20, 300) for _ in range(12)]
+    with capi.Server(t, u, wave, 1 << 20) as s:
+        slots = 2 * (wave + 4096 + capi.TEXT_PAD)
+        s.set_hbm_budget(slots)
+        alone = s.request(capi.REQ_LOOKUP, big)
+        r_alone, _ = s.wait(s.submit(alone))
+        made = [s.request(capi.REQ_COUNT, x) for x in smalls[:6]] + [s.request(capi.REQ_LOOKUP, big)] + \
+               [s.request(capi.REQ_COUNT, x) for x in smalls[6:]]
+        results = [s.wait(tk) for tk in [s.submit(r) for r in made]]
+        stats = s.stats()
+    assert r_alone["status"] == capi.KACHE_ENOMEM
+    assert results[6][0]["status"] == capi.KACHE_ENOMEM
+    assert stats["refused"] == 2
+    for i, (res, out) in enumerate(results):
+        if i == 6:
+            continue
+        assert res["status"] == capi.KACHE_OK
+        seqs = smalls[i if i < 6 else i - 1]
+        assert np.array_equal(out["counts"][:len(seqs)], o.counts(seqs)[0])
+    # Without the budget the same request runs.
+    with capi.Server(t, u, wave, 1 << 20) as s:
+        r, out = s.wait(s.submit(s.request(capi.REQ_LOOKUP, big)))
+    assert r["status"] == capi.KACHE_OK
+    f, v, fw = o.lookup(big)
+    p = len(f)
+    assert np.array_equal(out["found"][:p], f) and np.array_equal(out["fwd"][:p], fw)
+    t.close()

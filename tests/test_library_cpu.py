@@ -147,6 +147,7 @@ def test_server_rejects_invalid_arguments(api):
     assert L.kache_amd_server_submit(None, C.byref(r), C.byref(tk)) == E
     assert L.kache_amd_server_wait(None, 1, None) == E
     assert L.kache_amd_server_get_stats(None, None) == E
+    assert L.kache_amd_server_set_hbm_budget(None, 1) == E
     assert L.kache_amd_server_replay(None, None, None, 0, None) == E
     assert L.kache_amd_clock() > 0
 
