@@ -186,22 +186,29 @@ typedef struct kache_amd_tuning
     uint32_t launch_log2;
     uint32_t graph;
     uint32_t tuning;            /* get: rounds left to the tuner, 0 when done */
+    uint32_t waves_per_simd;    /* occupancy shape of the insert kernel: 6, 7 or 8 (0 on set: the default, 7) */
 } kache_amd_tuning;
 int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) KACHE_NOEXCEPT;
 int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) KACHE_NOEXCEPT;
 
-/* Online autotuner: the next insert launches of those calls try each
- * candidate shape (blocks per CU: all, 3/4, 1/2 of what fits; positions per
- * launch 2^23, 2^25; graph replay off, on) in turn, `rounds` launches each,
- * timed on the host; then the shape with the least median time per position
- * stays.  The launches are real work: nothing is repeated.  rounds = 0
- * stops it; at most 16. */
+/* Online autotuner of the insert launches of device-text calls: a
+ * coordinate descent over the launch shape, one dimension per stage in the
+ * order waves per SIMD the insert kernel is built for (6, 7, 8), blocks per
+ * CU (all that fit, 3/4, 1/2), positions per launch (2^23, 2^24, 2^25) and
+ * graph replay (off, on).  A stage's launches alternate the best shape so far
+ * (the reference) with each candidate, `rounds` times; a candidate's score
+ * is the median ratio of its host-timed time per position to the mean of the
+ * reference launches either side of it (which cancels the drift of the
+ * table's fill and new-key share during tuning), and the best candidate
+ * below 0.99 becomes the reference.  The launches are real work: nothing is
+ * repeated.  rounds = 0 stops it; at most 16. */
 int kache_amd_autotune(kache_table* t, uint32_t rounds) KACHE_NOEXCEPT;
 typedef struct kache_amd_tuning_trial
 {
     kache_amd_tuning cfg;
     uint32_t launches;
     double median_ns_per_position;
+    double median_vs_reference;     /* its score (1 for a stage's reference; 0 if not yet scored) */
 } kache_amd_tuning_trial;
 int kache_amd_tuning_trials(const kache_table* t, kache_amd_tuning_trial* out, size_t cap, size_t* n) KACHE_NOEXCEPT;
 

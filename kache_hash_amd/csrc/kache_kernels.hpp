@@ -77,8 +77,10 @@ __device__ __forceinline__ void hand_off(const Handoff& h)
     }
 }
 
-template <bool MAP, uint32_t OP, bool ORIENT>
-__global__ KAMD_INSERT_BOUNDS void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
+// WAVES: the waves per SIMD its registers must allow (the occupancy shape
+// the launch tuner chooses among).
+template <bool MAP, uint32_t OP, bool ORIENT, int WAVES = KAMD_INSERT_WAVES>
+__global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
 {
     hand_off(prev);
     __shared__ WaveLds wave_lds[TILE / 64];

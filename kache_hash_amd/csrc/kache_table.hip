@@ -1166,29 +1166,40 @@ struct kache_table
 
     // --- Inserts ---
 
-    // The insert kernel of an operation.
-    const void* insert_kernel(uint32_t op, bool oriented) const
+    // The insert kernel of an operation, at `waves` waves per SIMD.
+    template <int WV>
+    const void* insert_kernel_at(uint32_t op, bool oriented) const
     {
         const auto f = [](auto kern) { return reinterpret_cast<const void*>(kern); };
         if(!map)
-            return f(k_insert<false, OP_KEEP, false>);
+            return f(k_insert<false, OP_KEEP, false, WV>);
         if(oriented)
             switch(op)
             {
-            case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, true>);
-            case OP_ADD:    return f(k_insert<true, OP_ADD, true>);
-            case OP_OR:     return f(k_insert<true, OP_OR, true>);
-            default:        return f(k_insert<true, OP_MAX, true>);
+            case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, true, WV>);
+            case OP_ADD:    return f(k_insert<true, OP_ADD, true, WV>);
+            case OP_OR:     return f(k_insert<true, OP_OR, true, WV>);
+            default:        return f(k_insert<true, OP_MAX, true, WV>);
             }
         switch(op)
         {
-        case OP_KEEP:   return f(k_insert<true, OP_KEEP, false>);
-        case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, false>);
-        case OP_ADD:    return f(k_insert<true, OP_ADD, false>);
-        case OP_OR:     return f(k_insert<true, OP_OR, false>);
-        case OP_MIN:    return f(k_insert<true, OP_MIN, false>);
-        case OP_MAX:    return f(k_insert<true, OP_MAX, false>);
-        default:        return f(k_insert<true, OP_ADD_WRAP, false>);
+        case OP_KEEP:   return f(k_insert<true, OP_KEEP, false, WV>);
+        case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, false, WV>);
+        case OP_ADD:    return f(k_insert<true, OP_ADD, false, WV>);
+        case OP_OR:     return f(k_insert<true, OP_OR, false, WV>);
+        case OP_MIN:    return f(k_insert<true, OP_MIN, false, WV>);
+        case OP_MAX:    return f(k_insert<true, OP_MAX, false, WV>);
+        default:        return f(k_insert<true, OP_ADD_WRAP, false, WV>);
+        }
+    }
+
+    const void* insert_kernel(uint32_t op, bool oriented, uint32_t waves = 0) const
+    {
+        switch(waves ? waves : insert_waves)
+        {
+        case 6:  return insert_kernel_at<6>(op, oriented);
+        case 8:  return insert_kernel_at<8>(op, oriented);
+        default: return insert_kernel_at<KAMD_INSERT_WAVES>(op, oriented);
         }
     }
 
@@ -1239,16 +1250,30 @@ struct kache_table
     // arguments.
     uint32_t launch_log2 = 24;
     bool use_graph = false;
+    uint32_t insert_waves = KAMD_INSERT_WAVES;   // occupancy shape of the insert kernel (6, 7 or 8 waves per SIMD)
 
+    // The tuner: a coordinate descent over the shape's dimensions (waves per
+    // SIMD, blocks per CU, positions per launch, graph replay).  A stage
+    // varies one dimension of the best shape so far; its launches alternate
+    // that shape (the reference) and each candidate, `tune_rounds` times, and
+    // a candidate's score is the median ratio of its time per position to
+    // the mean of the reference launches either side of it.  The ratio
+    // cancels what changes during tuning besides the shape (the table's
+    // fill, the share of new keys), which differs more between consecutive
+    // launches than the shapes do.  A candidate replaces the reference if its
+    // score is below 0.99.  Launches during tuning are real work.
     struct Trial
     {
         kache_amd_tuning cfg;
-        std::vector<double> ns_per_pos;
+        std::vector<double> ns_per_pos, rel;
     };
-    std::vector<Trial> trials;
+    std::vector<Trial> trials;          // every shape tried; the first of each stage is its reference
     uint32_t tune_rounds = 0;           // > 0 while tuning
-    kache_amd_tuning tuned_from{};
-    size_t tune_next = 0;
+    uint32_t tune_stage = 0;
+    std::vector<int> tune_plan;         // the stage's launches: trial indices
+    size_t tune_next = 0;               // next launch of the plan
+    std::vector<std::pair<int, double>> tune_seen;   // the stage's launches so far: (trial, ns per position)
+    kache_amd_tuning tune_best{};
 
     // One graph per insert kernel: [kernel, counters to host and zeroed].
     struct InsertGraph
@@ -1331,10 +1356,12 @@ struct kache_table
     Counters insert_launch(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr)
     {
         Trial* trial = nullptr;
+        int trial_i = -1;
         if(tune_rounds)
         {
-            trial = &trials[tune_next % trials.size()];
-            blocks_per_cu = trial->cfg.blocks_per_cu, use_graph = trial->cfg.graph != 0;
+            trial_i = tune_plan[tune_next];
+            trial = &trials[trial_i];
+            apply_shape(trial->cfg);
         }
         const auto t0 = std::chrono::steady_clock::now();
         Counters c;
@@ -1351,48 +1378,117 @@ struct kache_table
         }
         if(trial)
         {
-            trial->ns_per_pos.push_back(std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / double(e - b));
-            if(++tune_next == trials.size() * tune_rounds)
-                finish_tuning();
+            const double ns = std::chrono::duration<double, std::nano>(std::chrono::steady_clock::now() - t0).count() / double(e - b);
+            trial->ns_per_pos.push_back(ns);
+            tune_seen.push_back({trial_i, ns});
+            if(++tune_next == tune_plan.size())
+                end_stage();
         }
         return c;
     }
 
-    // Candidates of the tuner: blocks per CU x graph replay, each tried on
-    // `rounds` launches in turn; launch sizes are tried likewise.
+    void apply_shape(const kache_amd_tuning& cfg)
+    {
+        blocks_per_cu = cfg.blocks_per_cu, use_graph = cfg.graph != 0, launch_log2 = cfg.launch_log2;
+        insert_waves = cfg.waves_per_simd ? cfg.waves_per_simd : KAMD_INSERT_WAVES;
+    }
+
+    // The shape the next launch takes: the tuner's while it runs.
+    uint32_t next_launch_log2() const { return tune_rounds ? trials[tune_plan[tune_next]].cfg.launch_log2 : launch_log2; }
+
     void start_tuning(uint32_t rounds)
     {
-        trials.clear();
-        tune_next = 0;
+        trials.clear(), tune_seen.clear(), tune_plan.clear();
+        tune_next = 0, tune_stage = 0;
         tune_rounds = rounds;
         if(!rounds)
             return;
-        tuned_from = kache_amd_tuning{blocks_per_cu, launch_log2, use_graph ? 1u : 0u, 0};
-        int resident = 0;
-        if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, insert_kernel(OP_ADD, false), TILE, 0) != hipSuccess || resident < 1)
-            resident = 8;
-        std::vector<uint32_t> blocks{0};
-        for(uint32_t b : {uint32_t(resident * 3 / 4), uint32_t(resident / 2)})
-            if(b >= 1 && b != static_cast<uint32_t>(resident) && std::find(blocks.begin(), blocks.end(), b) == blocks.end())
-                blocks.push_back(b);
-        for(uint32_t lg : {23u, 25u})
-            for(uint32_t b : blocks)
-                for(uint32_t g : {0u, 1u})
-                    trials.push_back({kache_amd_tuning{b, lg, g, 0}, {}});
+        tune_best = kache_amd_tuning{blocks_per_cu, launch_log2, use_graph ? 1u : 0u, 0, insert_waves};
+        begin_stage();
+    }
+
+    // Plans the launches of the current stage (the reference, then each
+    // candidate after a reference, `tune_rounds` times, then a last reference).
+    void begin_stage()
+    {
+        std::vector<kache_amd_tuning> cands;
+        kache_amd_tuning c = tune_best;
+        switch(tune_stage)
+        {
+        case 0:
+            for(uint32_t wv : {6u, 7u, 8u})
+                if(wv != tune_best.waves_per_simd)
+                    c.waves_per_simd = wv, cands.push_back(c);
+            break;
+        case 1:
+        {
+            int resident = 0;
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, insert_kernel(OP_ADD, false, tune_best.waves_per_simd), TILE, 0) != hipSuccess ||
+               resident < 1)
+                resident = 8;
+            for(uint32_t b : {0u, static_cast<uint32_t>(resident * 3 / 4), static_cast<uint32_t>(resident / 2)})
+                if(b != tune_best.blocks_per_cu && (b == 0 || b < static_cast<uint32_t>(resident)))
+                    c.blocks_per_cu = b, cands.push_back(c);
+            break;
+        }
+        case 2:
+            for(uint32_t lg : {23u, 24u, 25u})
+                if(lg != tune_best.launch_log2)
+                    c.launch_log2 = lg, cands.push_back(c);
+            break;
+        case 3:
+            c.graph = tune_best.graph ? 0u : 1u, cands.push_back(c);
+            break;
+        default:
+            break;
+        }
+        if(cands.empty())
+        {
+            finish_tuning();
+            return;
+        }
+        const int ref = static_cast<int>(trials.size());
+        trials.push_back({tune_best, {}, {}});
+        for(const auto& x : cands)
+            trials.push_back({x, {}, {}});
+        tune_plan.clear(), tune_seen.clear();
+        tune_next = 0;
+        for(uint32_t r = 0; r < tune_rounds; ++r)
+            for(size_t i = 0; i < cands.size(); ++i)
+                tune_plan.push_back(ref), tune_plan.push_back(ref + 1 + static_cast<int>(i));
+        tune_plan.push_back(ref);
+    }
+
+    // Scores the stage's candidates against the reference launches beside
+    // them, keeps the best shape, and starts the next stage.
+    void end_stage()
+    {
+        const int ref = tune_plan.front();
+        for(size_t k = 1; k + 1 < tune_seen.size(); ++k)
+            if(tune_seen[k].first != ref)
+                trials[tune_seen[k].first].rel.push_back(tune_seen[k].second / (0.5 * (tune_seen[k - 1].second + tune_seen[k + 1].second)));
+        trials[ref].rel.assign(1, 1.0);
+        double best = 0.99;
+        for(size_t i = ref + 1; i < trials.size(); ++i)
+        {
+            std::vector<double> v = trials[i].rel;
+            if(v.empty())
+                continue;
+            std::sort(v.begin(), v.end());
+            const double med = v[v.size() / 2];
+            if(med < best)
+                best = med, tune_best = trials[i].cfg;
+        }
+        tune_stage++;
+        begin_stage();
     }
 
     void finish_tuning()
     {
-        double best = 0;
-        for(const auto& t : trials)
-        {
-            std::vector<double> v = t.ns_per_pos;
-            std::sort(v.begin(), v.end());
-            const double med = v.empty() ? 0 : v[v.size() / 2];
-            if(!v.empty() && (best == 0 || med < best))
-                best = med, blocks_per_cu = t.cfg.blocks_per_cu, use_graph = t.cfg.graph != 0, launch_log2 = t.cfg.launch_log2;
-        }
         tune_rounds = 0;
+        tune_plan.clear(), tune_seen.clear();
+        tune_best.tuning = 0;
+        apply_shape(tune_best);
     }
 
     // Inserts the positions of a device text, in launches of at most
@@ -1404,9 +1500,7 @@ struct kache_table
         uint64_t inserted = 0;
         for(uint64_t b = first; b < text_len;)
         {
-            if(tune_rounds)
-                launch_log2 = trials[tune_next % trials.size()].cfg.launch_log2;
-            const uint64_t e = std::min(text_len, b + (uint64_t(1) << launch_log2));
+            const uint64_t e = std::min(text_len, b + (uint64_t(1) << next_launch_log2()));
             ov_room(e - b);
             if(size + 2 * (e - b) >= resize_th)
                 prepare_resize();
@@ -3016,10 +3110,12 @@ int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) noexcept
 
 int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) noexcept
 {
-    if(t == nullptr || cfg == nullptr || cfg->blocks_per_cu > 64 || cfg->launch_log2 < 16 || cfg->launch_log2 > 30 || cfg->graph > 1)
+    if(t == nullptr || cfg == nullptr || cfg->blocks_per_cu > 64 || cfg->launch_log2 < 16 || cfg->launch_log2 > 30 || cfg->graph > 1 ||
+       !(cfg->waves_per_simd == 0 || (cfg->waves_per_simd >= 6 && cfg->waves_per_simd <= 8)))
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
     t->blocks_per_cu = cfg->blocks_per_cu, t->launch_log2 = cfg->launch_log2, t->use_graph = cfg->graph != 0;
+    t->insert_waves = cfg->waves_per_simd ? cfg->waves_per_simd : KAMD_INSERT_WAVES;
     t->tune_rounds = 0;
     return KACHE_OK;
 }
@@ -3029,7 +3125,7 @@ int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) noexcept
     if(t == nullptr || cfg == nullptr)
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
-    *cfg = kache_amd_tuning{t->blocks_per_cu, t->launch_log2, t->use_graph ? 1u : 0u, t->tune_rounds};
+    *cfg = kache_amd_tuning{t->blocks_per_cu, t->launch_log2, t->use_graph ? 1u : 0u, t->tune_rounds, t->insert_waves};
     return KACHE_OK;
 }
 
@@ -3049,9 +3145,11 @@ int kache_amd_tuning_trials(const kache_table* t, kache_amd_tuning_trial* out, s
     *n = t->trials.size();
     for(size_t i = 0; i < std::min(cap, t->trials.size()); ++i)
     {
-        std::vector<double> v = t->trials[i].ns_per_pos;
+        std::vector<double> v = t->trials[i].ns_per_pos, r = t->trials[i].rel;
         std::sort(v.begin(), v.end());
-        out[i] = kache_amd_tuning_trial{t->trials[i].cfg, static_cast<uint32_t>(v.size()), v.empty() ? 0.0 : v[v.size() / 2]};
+        std::sort(r.begin(), r.end());
+        out[i] = kache_amd_tuning_trial{t->trials[i].cfg, static_cast<uint32_t>(v.size()), v.empty() ? 0.0 : v[v.size() / 2],
+                                        r.empty() ? 0.0 : r[r.size() / 2]};
     }
     return KACHE_OK;
 }

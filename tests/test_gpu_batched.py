@@ -84,8 +84,8 @@ def run_shape(api, d, cuts, width, shape, expected, op, oriented):
 SHAPES = [
     None,
     {"blocks_per_cu": 0, "launch_log2": 16, "graph": False},
-    {"blocks_per_cu": 3, "launch_log2": 17, "graph": False},
-    {"blocks_per_cu": 0, "launch_log2": 16, "graph": True},
+    {"blocks_per_cu": 3, "launch_log2": 17, "graph": False, "waves_per_simd": 6},
+    {"blocks_per_cu": 0, "launch_log2": 16, "graph": True, "waves_per_simd": 8},
     {"blocks_per_cu": 2, "launch_log2": 18, "graph": True},
     "autotune",
 ]
@@ -116,7 +116,7 @@ def test_launch_shapes_give_the_same_table(api, width, oriented):
         assert cap == ref, shape
         if shape == "autotune":
             trials = t.tuning_trials()
-            assert len(trials) >= 4 and all(tr["launches"] <= 1 for tr in trials)
+            assert len(trials) >= 3 and {tr["waves_per_simd"] for tr in trials[:3]} == {6, 7, 8}
         t.close()
 
 
@@ -128,8 +128,14 @@ def test_tuning_validation(api):
         t.set_tuning(0, 31, False)
     with pytest.raises(capi.KacheError):
         t.autotune(17)
+    with pytest.raises(capi.KacheError):
+        t.set_tuning(0, 20, False, 5)
+    with pytest.raises(capi.KacheError):
+        t.set_tuning(0, 20, False, 9)
     t.set_tuning(4, 20, True)
-    assert t.tuning() == {"blocks_per_cu": 4, "launch_log2": 20, "graph": 1, "tuning": 0}
+    assert t.tuning() == {"blocks_per_cu": 4, "launch_log2": 20, "graph": 1, "tuning": 0, "waves_per_simd": 7}
+    t.set_tuning(0, 22, False, 8)
+    assert t.tuning() == {"blocks_per_cu": 0, "launch_log2": 22, "graph": 0, "tuning": 0, "waves_per_simd": 8}
     # Batch calls check their bounds before reading them.
     u = t.register()
     d = capi.DeviceBuffer(api, 64 + capi.TEXT_PAD)
