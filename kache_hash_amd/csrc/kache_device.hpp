@@ -300,14 +300,27 @@ __device__ __forceinline__ Res probe_insert(uint8_t* bucket, uint64_t sk, uint8_
 template <bool MAP>
 __device__ __forceinline__ uint64_t* probe_find(uint8_t* bucket, uint64_t sk, uint8_t c, bool& full)
 {
-    uint32_t eq, zero;
-    cs_masks(bucket, c, eq, zero);
-    full = !(zero >> 31);
-    while(eq)
+    // Checksum word i's equal bytes b flag bit 8b + i of m (slot 4i + b): the
+    // zero-byte test leaves 0x80 in each, moved down by 7 - i, with no
+    // multiply to gather them.  Candidates come in no particular slot order,
+    // which is fine here: a key sits in at most one slot of a bucket.
+    const uint4* p = reinterpret_cast<const uint4*>(bucket);
+    const uint4 a = p[0], b = p[1];
+    const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    const uint32_t cc = 0x01010101u * c;
+    uint32_t m = 0;
+#pragma unroll
+    for(int i = 0; i < 8; ++i)
     {
-        const uint32_t j = __builtin_ctz(eq);
-        eq &= eq - 1;
-        uint64_t* s = slot_at<MAP>(bucket, j);
+        const uint32_t y = d[i] ^ cc;
+        m |= (~(((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y | 0x7F7F7F7Fu)) >> (7 - i);
+    }
+    full = (d[7] >> 24) != 0;       // slots fill in order: the last one taken
+    while(m)
+    {
+        const uint32_t bit = __builtin_ctz(m);
+        m &= m - 1;
+        uint64_t* s = slot_at<MAP>(bucket, 4 * (bit & 7) + (bit >> 3));
         if(*s == sk)
             return s;
     }
