@@ -39,6 +39,24 @@ ALG_BYTES_PER_KMER = 150 / 120 + 64 / 5.29 + 16
 LINES_PER_KMER = 1 / 5.29 + 16 / 64
 
 
+def transaction_roof(kmers_per_s):
+    """The insert kernel against the device's random-transaction capacity:
+    its memory reads and memory-side atomics per k-mer (rocprofv3 counters,
+    profiles/insert_count_map_traffic.json) at the measured k-mers/s, each
+    over its measured roof (random_read of 128-byte lines, random_atomic;
+    profiles/memory_roof_mi355x.json), summed."""
+    try:
+        tr = json.load(open(os.path.join(ROOT, "profiles", "insert_count_map_traffic.json")))
+        roof = json.load(open(os.path.join(ROOT, "profiles", "memory_roof_mi355x.json")))
+        reads, atomics = tr["memory_reads_per_kmer"], tr["memory_atomics_per_kmer"]
+        r_roof, a_roof = roof["random_read_128_lines_per_s"], roof["random_atomic_64_lines_per_s"]
+    except (OSError, KeyError, ValueError):
+        return None
+    r, a = kmers_per_s * reads, kmers_per_s * atomics
+    return {"reads_per_kmer": reads, "atomics_per_kmer": atomics, "reads_per_s": r, "atomics_per_s": a,
+            "read_roof_per_s": r_roof, "atomic_roof_per_s": a_roof, "frac": r / r_roof + a / a_roof}
+
+
 def random_line_roof(kmers_per_s):
     """The insert kernel against the measured random read-modify-write roof of
     64-byte lines (profiles/memory_roof_mi355x.json, made by
@@ -334,7 +352,8 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "k_insert<map, ADD>", "alg_bytes_per_kmer": ALG_BYTES_PER_KMER,
                      "kmers_per_launch": per_launch_kmers, "avg_launch_ms": avg_ms,
-                     "random_line_roof": random_line_roof(per_launch_kmers / (avg_ms * 1e-3) if avg_ms > 0 else 0.0)},
+                     "random_line_roof": random_line_roof(per_launch_kmers / (avg_ms * 1e-3) if avg_ms > 0 else 0.0),
+                     "transaction_roof": transaction_roof(per_launch_kmers / (avg_ms * 1e-3) if avg_ms > 0 else 0.0)},
         "cpu_baseline": cpu,
         "served": served,
     }
