@@ -216,7 +216,7 @@ int kache_amd_tuning_trials(const kache_table* t, kache_amd_tuning_trial* out, s
 typedef struct kache_amd_info
 {
     uint64_t buckets;
-    uint64_t bucket_bytes;
+    uint64_t bucket_bytes;      /* 320 (set), 448 (values of 1-4 bytes, in 4-byte words), 576 (8-byte values) */
     uint64_t overflow_slots;
     uint64_t overflow_keys;
     uint64_t resizes;

@@ -6,15 +6,19 @@
 //   bucket b, at buckets + b * bucket_bytes:
 //     [ 0, 32)  checksum of slot j (0 = empty)          include/Streaming_Kmer_Hash_Table.hpp:91-97
 //     [32, 64)  flag bits 1..255: the spill filter       (:1063-1093)
-//     [64, ..)  32 slots: stored key (~key, 0 = empty) [+ value u64]
-//   overflow: power-of-two array of slots {~key [, value]}, linear probing from
+//     [64, 320) 32 stored keys (~key, 0 = empty)
+//     [320, ..) in a map, the 32 values: 4-byte words for widths 1-4
+//               (448-byte buckets), 8-byte words for widths 5-8 (576 bytes)
+//   overflow: power-of-two array of slots in blocks of 32, each block its 32
+//   stored keys then (in a map) their 32 values, linear probing from
 //   XXH3_64bits(key) (include/Concurrent_Hash_Table.hpp:28, 43-52, 205-300).
-// Values of every width sit zero-extended in a u64, so one 64-bit atomic
-// updates any of them.
+// A value sits zero-extended in its word, so one atomic of the word's size
+// updates it.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace kamd
 {
@@ -180,17 +184,21 @@ __host__ __device__ __forceinline__ void pack8(uint64_t v, uint32_t& code, uint3
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
 
 __device__ __forceinline__ uint64_t ld_acq(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ bool cas64(uint64_t* p, uint64_t& expected, uint64_t desired)
 {
     return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t add64(uint64_t* p, uint64_t v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void or64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void and64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void min64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void max64(uint64_t* p, uint64_t v) { __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void or32(uint32_t* p, uint32_t v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Atomics on a value word (4 or 8 bytes).
+template <typename T> __device__ __forceinline__ void at_store(T* p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ T at_add(T* p, T v) { return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ void at_or(T* p, T v) { __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ void at_and(T* p, T v) { __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ void at_min(T* p, T v) { __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ void at_max(T* p, T v) { __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T> __device__ __forceinline__ void at_xchg(T* p, T v) { __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Orders the wave's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_sync()
@@ -235,70 +243,74 @@ struct TableView
     uint32_t width;         // value bytes (0 in a set)
 };
 
-template <bool MAP> struct Geo
+// Value bytes per slot, VB: 0 (a set), 4 (maps of widths 1-4) or 8 (widths
+// 5-8).  V is the value word.
+template <int VB> struct Geo
 {
-    static constexpr uint32_t slot_words = MAP ? 2 : 1;                 // u64 words per slot
-    static constexpr uint32_t bucket_bytes = META_BYTES + B * 8 * slot_words;
+    static_assert(VB == 0 || VB == 4 || VB == 8, "value words are 4 or 8 bytes");
+    using V = typename std::conditional<VB == 4, uint32_t, uint64_t>::type;
+    static constexpr uint32_t block_bytes = B * (8 + VB);               // 32 keys, then their 32 values
+    static constexpr uint32_t bucket_bytes = META_BYTES + block_bytes;
 };
 
-template <bool MAP>
-__device__ __forceinline__ uint8_t* bucket_at(const TableView& T, uint64_t b) { return T.buckets + b * Geo<MAP>::bucket_bytes; }
+// The layout of a table whose values are `width` bytes.
+__host__ __device__ constexpr uint32_t value_word(uint32_t width) { return width == 0 ? 0 : width <= 4 ? 4 : 8; }
 
-// Slots are split into words: a bucket holds its 32 stored keys, then (in a
-// map) their 32 values, so the value of the key at word k is word k + VAL_OFF.
-// The overflow table keeps the same shape in blocks of 32 slots.
-constexpr uint32_t VAL_OFF = B;
+template <int VB>
+__device__ __forceinline__ uint8_t* bucket_at(const TableView& T, uint64_t b) { return T.buckets + b * Geo<VB>::bucket_bytes; }
 
-template <bool MAP>
+// Keys and values are kept in separate arrays of a block of 32 slots (a
+// bucket past its metadata, or 32 overflow slots): the key of slot j at word
+// j, the values after the 32 keys.
+template <int VB>
 __device__ __forceinline__ uint64_t* slot_at(uint8_t* bucket, uint32_t j) { return reinterpret_cast<uint64_t*>(bucket + META_BYTES) + j; }
 
-template <bool MAP>
+template <int VB>
 __host__ __device__ __forceinline__ uint64_t* ov_slot(uint64_t* ov, uint64_t i)
 {
-    return MAP ? ov + (i & ~uint64_t(B - 1)) * 2 + (i & (B - 1)) : ov + i;
+    if constexpr(VB == 0)
+        return ov + i;
+    else if constexpr(VB == 8)
+        return ov + (i & ~uint64_t(B - 1)) * 2 + (i & (B - 1));
+    else
+        return ov + (i / B) * (Geo<VB>::block_bytes / 8) + (i & (B - 1));
 }
+
+// A slot as the probes hand it on: its key word, and where its value is.
+// With 8-byte values that is 32 words past the key; with 4-byte values
+// (block + 256 + 4j for slot j of a block) it depends on j, which the 4-byte
+// layout keeps beside the key pointer.
+template <int VB> struct Slot
+{
+    uint64_t* k;
+    Slot() = default;
+    __device__ __forceinline__ Slot(uint64_t* key, uint32_t) : k(key) {}
+    __device__ __forceinline__ uint32_t j() const { return 0; }
+    __device__ __forceinline__ typename Geo<VB>::V* v() const { return k + B; }
+};
+
+template <> struct Slot<4>
+{
+    uint64_t* k;
+    uint32_t jj;
+    Slot() = default;
+    __device__ __forceinline__ Slot(uint64_t* key, uint32_t j) : k(key), jj(j) {}
+    __device__ __forceinline__ uint32_t j() const { return jj; }
+    __device__ __forceinline__ uint32_t* v() const { return reinterpret_cast<uint32_t*>(k) + 2 * B - jj; }
+};
+
+template <int VB>
+__device__ __forceinline__ Slot<VB> slot_ref(uint8_t* bucket, uint32_t j) { return Slot<VB>(slot_at<VB>(bucket, j), j); }
+
+template <int VB>
+__device__ __forceinline__ Slot<VB> ov_ref(uint64_t* ov, uint64_t i) { return Slot<VB>(ov_slot<VB>(ov, i), static_cast<uint32_t>(i & (B - 1))); }
 
 enum Res : int { RES_FOUND = 0, RES_NEW = 1, RES_FULL = 2 };
 
-// Finds the stored key `sk` (checksum `c`) in the bucket, or claims its first
-// empty slot for it.  Slots fill in order and keys are never removed, so a
-// walk over the slots from the first empty one in order, claiming each empty
-// slot by CAS, finds a concurrent insertion of the same key wherever it went.
-// The checksums serve as hints only: they may lag their keys.
-template <bool MAP>
-__device__ __forceinline__ Res probe_insert(uint8_t* bucket, uint64_t sk, uint8_t c, uint64_t*& slot)
-{
-    uint32_t eq, zero;
-    cs_masks(bucket, c, eq, zero);
-    while(eq)
-    {
-        const uint32_t j = __builtin_ctz(eq);
-        eq &= eq - 1;
-        uint64_t* s = slot_at<MAP>(bucket, j);
-        if(ld_acq(s) == sk)
-            return slot = s, RES_FOUND;
-    }
-
-    for(uint32_t j = zero ? __builtin_ctz(zero) : B; j < B; ++j)
-    {
-        uint64_t* s = slot_at<MAP>(bucket, j);
-        uint64_t expected = 0;
-        if(cas64(s, expected, sk))
-        {
-            bucket[j] = c;
-            return slot = s, RES_NEW;
-        }
-        if(expected == sk)
-            return slot = s, RES_FOUND;
-    }
-
-    return RES_FULL;
-}
-
-// Read-only probe: the slot holding `sk`, or null; `full` iff the bucket's
-// last slot is taken (slots fill in order).
-template <bool MAP>
-__device__ __forceinline__ uint64_t* probe_find(uint8_t* bucket, uint64_t sk, uint8_t c, bool& full)
+// Read-only probe: the slot holding `sk` (a null key pointer if none);
+// `full` iff the bucket's last slot is taken (slots fill in order).
+template <int VB>
+__device__ __forceinline__ Slot<VB> probe_find(uint8_t* bucket, uint64_t sk, uint8_t c, bool& full)
 {
     // Checksum word i's equal bytes b flag bit 8b + i of m (slot 4i + b): the
     // zero-byte test leaves 0x80 in each, moved down by 7 - i, with no
@@ -320,27 +332,28 @@ __device__ __forceinline__ uint64_t* probe_find(uint8_t* bucket, uint64_t sk, ui
     {
         const uint32_t bit = __builtin_ctz(m);
         m &= m - 1;
-        uint64_t* s = slot_at<MAP>(bucket, 4 * (bit & 7) + (bit >> 3));
+        const uint32_t j = 4 * (bit & 7) + (bit >> 3);
+        uint64_t* s = slot_at<VB>(bucket, j);
         if(*s == sk)
-            return s;
+            return Slot<VB>(s, j);
     }
-    return nullptr;
+    return Slot<VB>(nullptr, 0);
 }
 
 // Overflow table: linear probing from XXH3_64bits(key) (include/Concurrent_Hash_Table.hpp:205-300).
-template <bool MAP>
-__device__ __forceinline__ Res ov_insert(const TableView& T, uint64_t key, uint64_t sk, uint64_t*& slot)
+template <int VB>
+__device__ __forceinline__ Res ov_insert(const TableView& T, uint64_t key, uint64_t sk, Slot<VB>& slot)
 {
     uint64_t i = xxh3_8(key, 0) & T.ov_mask;
     for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
     {
-        uint64_t* s = ov_slot<MAP>(T.ov, i);
-        uint64_t cur = ld_acq(s);
+        const Slot<VB> s = ov_ref<VB>(T.ov, i);
+        uint64_t cur = ld_acq(s.k);
         if(cur == sk)
             return slot = s, RES_FOUND;
         if(cur == 0)
         {
-            if(cas64(s, cur, sk))
+            if(cas64(s.k, cur, sk))
                 return slot = s, RES_NEW;
             if(cur == sk)
                 return slot = s, RES_FOUND;
@@ -349,20 +362,20 @@ __device__ __forceinline__ Res ov_insert(const TableView& T, uint64_t key, uint6
     return RES_FULL;
 }
 
-template <bool MAP>
-__device__ __forceinline__ uint64_t* ov_find(const TableView& T, uint64_t key, uint64_t sk)
+template <int VB>
+__device__ __forceinline__ Slot<VB> ov_find(const TableView& T, uint64_t key, uint64_t sk)
 {
     uint64_t i = xxh3_8(key, 0) & T.ov_mask;
     for(uint64_t n = 0; n <= T.ov_mask; ++n, i = (i + 1) & T.ov_mask)
     {
-        uint64_t* s = ov_slot<MAP>(T.ov, i);
-        const uint64_t cur = *s;
+        const Slot<VB> s = ov_ref<VB>(T.ov, i);
+        const uint64_t cur = *s.k;
         if(cur == sk)
             return s;
         if(cur == 0)
-            return nullptr;
+            return Slot<VB>(nullptr, 0);
     }
-    return nullptr;
+    return Slot<VB>(nullptr, 0);
 }
 
 // Marks the spill of minimizer hash `h` in the bucket's filter (:1079-1085).
@@ -376,35 +389,6 @@ __device__ __forceinline__ bool may_spill(const uint8_t* bucket, uint64_t h)
 {
     const uint32_t i = filter_pos(h);
     return (bucket[32 + (i >> 3)] >> (i & 7)) & 1u;
-}
-
-// Inserts with the reference's policy (include/Streaming_Kmer_Hash_Table.hpp:2431-2490):
-// the primary bucket; once it is full, the spill mark, then the secondaries
-// and the overflow table.  The secondaries go in a fixed order, p then q, so
-// that concurrent inserters of one key walk the same slots.  RES_FULL only if
-// the overflow table has no room.
-template <bool MAP>
-__device__ __forceinline__ Res insert_key(const TableView& T, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
-{
-    const uint64_t sk = ~key;
-    in_ov = false;
-    uint8_t* bb = bucket_at<MAP>(T, bucket_of(h, T.cap));
-    Res r = probe_insert<MAP>(bb, sk, c, slot);
-    if(r != RES_FULL)
-        return r;
-
-    mark_spill(bb, h);
-    uint64_t p, q;
-    secondaries(h, T.cap, p, q);
-    r = probe_insert<MAP>(bucket_at<MAP>(T, p), sk, c, slot);
-    if(r != RES_FULL)
-        return r;
-    r = probe_insert<MAP>(bucket_at<MAP>(T, q), sk, c, slot);
-    if(r != RES_FULL)
-        return r;
-
-    in_ov = true;
-    return ov_insert<MAP>(T, key, sk, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,26 +422,33 @@ __device__ __forceinline__ uint64_t exchange64(uint64_t* p, uint64_t v)
     return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Inserts with the policy of insert_key, for all lanes of a wave at once
-// (every lane calls it; `active` lanes have a key).  Lanes that need a new
-// slot in the same bucket, from the same lowest possibly-free slot z, take
-// consecutive slots z, z + 1, ... in one round of CASes, ranked by lane,
-// instead of contending for one slot at a time: the first claims with its
-// key (nothing below z can hold it: those slots' checksums were read), the
-// others with a reservation.  A reserving lane then reads the slots from z
-// below its own: all are taken (by the round's other lanes or by others
-// before), so the key can only be there as a key or a reservation; if it is,
-// the lowest such slot is the key's (its owner sees no copy below it) and the
-// lane turns its reservation into a tombstone; otherwise it writes its key
-// over the reservation.  Nothing waits on anything, so a wave never blocks
-// on another.  A lane whose CAS loses learns that the slot is taken and
-// tries again above it.
-template <bool MAP>
-__device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
+// Inserts with the reference's policy (include/Streaming_Kmer_Hash_Table.hpp:2431-2490):
+// the primary bucket; once it is full, the spill mark, then the secondaries
+// and the overflow table.  The secondaries go in a fixed order, p then q, so
+// that concurrent inserters of one key walk the same slots.  RES_FULL only if
+// the overflow table has no room (or after CLAIM_ROUNDS rounds).  Slots fill
+// in order and keys are never removed, so a walk over a bucket's slots from
+// its first empty one finds a concurrent insertion of the same key wherever it
+// went; the checksums serve as hints only (they may lag their keys).  All lanes
+// of a wave insert at once (every lane calls it; `active` lanes have a key).
+// Lanes that need a new slot in the same bucket, from the same lowest
+// possibly-free slot z, take consecutive slots z, z + 1, ... in one round of
+// CASes, ranked by lane, instead of contending for one slot at a time: the
+// first claims with its key (nothing below z can hold it: those slots'
+// checksums were read), the others with a reservation.  A reserving lane then
+// reads the slots from z below its own: all are taken (by the round's other
+// lanes or by others before), so the key can only be there as a key or a
+// reservation; if it is, the lowest such slot is the key's (its owner sees no
+// copy below it) and the lane turns its reservation into a tombstone;
+// otherwise it writes its key over the reservation.  Nothing waits on
+// anything, so a wave never blocks on another.  A lane whose CAS loses learns
+// that the slot is taken and tries again above it.
+template <int VB>
+__device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, Slot<VB>& slot, bool& in_ov)
 {
     const uint64_t sk = ~key;
     in_ov = false;
-    slot = nullptr;
+    slot = Slot<VB>(nullptr, 0);
     Res r = RES_FOUND;
     // Per-lane state: the policy's stage (0 primary, 1 p, 2 q) and the lowest
     // slot of the current bucket not known to be taken.
@@ -472,7 +463,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
                 r = RES_FULL, need = false;
             break;
         }
-        uint8_t* bk = bucket_at<MAP>(T, bidx);
+        uint8_t* bk = bucket_at<VB>(T, bidx);
         uint32_t z = B;
         if(need)
         {
@@ -482,10 +473,10 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             {
                 const uint32_t j = __builtin_ctz(eq);
                 eq &= eq - 1;
-                uint64_t* s = slot_at<MAP>(bk, j);
+                uint64_t* s = slot_at<VB>(bk, j);
                 if(ld_acq(s) == sk)
                 {
-                    slot = s, r = RES_FOUND, need = false;
+                    slot = Slot<VB>(s, j), r = RES_FOUND, need = false;
                     break;
                 }
             }
@@ -536,7 +527,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             bool mine = false;
             uint64_t occ = 0;
             if(j < B)
-                mine = cas64(slot_at<MAP>(bk, j), occ, rank == 0 ? sk : rk);
+                mine = cas64(slot_at<VB>(bk, j), occ, rank == 0 ? sk : rk);
             // Slots z .. j-1 were the targets of this group's lower-ranked
             // lanes: those that won them hold other keys, so only a slot
             // whose CAS lost can hold a concurrent claim of this key.
@@ -544,7 +535,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
             if(mine && rank == 0)
             {
                 bk[j] = c;
-                slot = slot_at<MAP>(bk, j), r = RES_NEW, need = false;
+                slot = slot_ref<VB>(bk, j), r = RES_NEW, need = false;
             }
             else
             {
@@ -553,7 +544,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
                 const uint32_t top = (grp & lost & below_me) ? (j < B ? j : B) : z;
                 for(uint32_t t = z; t < top; ++t)
                 {
-                    const uint64_t v = ld_acq(slot_at<MAP>(bk, t));
+                    const uint64_t v = ld_acq(slot_at<VB>(bk, t));
                     if(v == sk || v == rk)
                     {
                         cand = t;
@@ -564,15 +555,15 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
                     cand = j;
                 if(cand < B)
                 {
-                    if(mine && exchange64(slot_at<MAP>(bk, j), TOMB) == rk)
+                    if(mine && exchange64(slot_at<VB>(bk, j), TOMB) == rk)
                         bk[j] = c;
-                    slot = slot_at<MAP>(bk, cand), r = RES_FOUND, need = false;
+                    slot = slot_ref<VB>(bk, cand), r = RES_FOUND, need = false;
                 }
                 else if(mine)
                 {
-                    if(exchange64(slot_at<MAP>(bk, j), sk) == rk)
+                    if(exchange64(slot_at<VB>(bk, j), sk) == rk)
                         bk[j] = c;
-                    slot = slot_at<MAP>(bk, j), r = RES_NEW, need = false;
+                    slot = slot_ref<VB>(bk, j), r = RES_NEW, need = false;
                 }
                 else if(j >= B)
                 {
@@ -589,7 +580,7 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
                     else
                     {
                         in_ov = true;
-                        r = ov_insert<MAP>(T, ~sk, sk, slot);
+                        r = ov_insert<VB>(T, ~sk, sk, slot);
                         need = false;
                     }
                 }
@@ -601,21 +592,22 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
         {
             // Followers take their owner's outcome, or its next step.
             const int src = owner >= 0 ? owner : static_cast<int>(__lane_id());
+            // State bits: need 0, r 1-2, in_ov 3, stage 4-7, lo 8-15, the slot's j 16-20.
             const uint32_t o_state = __shfl(static_cast<int>(need) | (static_cast<int>(r) << 1) | (static_cast<int>(in_ov) << 3) |
-                                            static_cast<int>(stage << 4) | static_cast<int>(lo << 8), src, 64);
-            const uint64_t o_slot = __shfl(static_cast<unsigned long long>(reinterpret_cast<uintptr_t>(slot)), src, 64);
+                                            static_cast<int>(stage << 4) | static_cast<int>(lo << 8) | static_cast<int>(slot.j() << 16), src, 64);
+            const uint64_t o_slot = __shfl(static_cast<unsigned long long>(reinterpret_cast<uintptr_t>(slot.k)), src, 64);
             const uint64_t o_b = __shfl(static_cast<unsigned long long>(bidx), src, 64);
             if(owner >= 0)
             {
                 if(!(o_state & 1))
                 {
-                    slot = reinterpret_cast<uint64_t*>(static_cast<uintptr_t>(o_slot));
+                    slot = Slot<VB>(reinterpret_cast<uint64_t*>(static_cast<uintptr_t>(o_slot)), (o_state >> 16) & (B - 1));
                     r = ((o_state >> 1) & 3) == RES_FULL ? RES_FULL : RES_FOUND;
                     in_ov = (o_state >> 3) & 1;
                     need = false;
                 }
                 else
-                    bidx = o_b, stage = (o_state >> 4) & 15, lo = o_state >> 8;
+                    bidx = o_b, stage = (o_state >> 4) & 15, lo = (o_state >> 8) & (VB == 4 ? 255u : ~0u);
             }
         }
     }
@@ -631,12 +623,12 @@ __device__ __forceinline__ Res insert_key_wave(const TableView& T, bool active, 
 // into keys, no slots below to scan.  A lost CAS means the slot went to
 // another key: the lane tries again above it.  Same policy, same result
 // (RES_NEW, or RES_FULL if the overflow table has no room).
-template <bool MAP>
-__device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, uint64_t*& slot, bool& in_ov)
+template <int VB>
+__device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, uint64_t key, uint64_t h, uint8_t c, Slot<VB>& slot, bool& in_ov)
 {
     const uint64_t sk = ~key;
     in_ov = false;
-    slot = nullptr;
+    slot = Slot<VB>(nullptr, 0);
     Res r = RES_NEW;
     uint32_t stage = 0, lo = 0;
     uint64_t bidx = active ? bucket_of(h, T.cap) : 0;
@@ -649,7 +641,7 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
                 r = RES_FULL, need = false;
             break;
         }
-        uint8_t* bk = bucket_at<MAP>(T, bidx);
+        uint8_t* bk = bucket_at<VB>(T, bidx);
         uint32_t z = B;
         if(need)
         {
@@ -675,10 +667,10 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
             if(j < B)
             {
                 uint64_t occ = 0;
-                if(cas64(slot_at<MAP>(bk, j), occ, sk))
+                if(cas64(slot_at<VB>(bk, j), occ, sk))
                 {
                     bk[j] = c;
-                    slot = slot_at<MAP>(bk, j), need = false;
+                    slot = slot_ref<VB>(bk, j), need = false;
                 }
                 else
                     lo = j + 1;
@@ -698,7 +690,7 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
                 else
                 {
                     in_ov = true;
-                    r = ov_insert<MAP>(T, key, sk, slot);
+                    r = ov_insert<VB>(T, key, sk, slot);
                     need = false;
                 }
             }
@@ -710,78 +702,84 @@ __device__ __forceinline__ Res insert_new_wave(const TableView& T, bool active, 
 // Lookup policy (include/Streaming_Kmer_Hash_Table.hpp:2548-2574): the
 // primary; absent unless it is full and its filter marks the minimizer; the
 // secondaries; the overflow table iff both are full.
-template <bool MAP>
-__device__ __forceinline__ uint64_t* find_key(const TableView& T, uint64_t key, uint64_t h, uint8_t c)
+template <int VB>
+__device__ __forceinline__ Slot<VB> find_key(const TableView& T, uint64_t key, uint64_t h, uint8_t c)
 {
     const uint64_t sk = ~key;
-    uint8_t* bb = bucket_at<MAP>(T, bucket_of(h, T.cap));
+    uint8_t* bb = bucket_at<VB>(T, bucket_of(h, T.cap));
     bool full;
-    uint64_t* s = probe_find<MAP>(bb, sk, c, full);
-    if(s || !full || !may_spill(bb, h))
+    const Slot<VB> s = probe_find<VB>(bb, sk, c, full);
+    if(s.k || !full || !may_spill(bb, h))
         return s;
 
     uint64_t p, q;
     secondaries(h, T.cap, p, q);
     bool full_p, full_q;
-    if((s = probe_find<MAP>(bucket_at<MAP>(T, p), sk, c, full_p)))
-        return s;
-    if((s = probe_find<MAP>(bucket_at<MAP>(T, q), sk, c, full_q)))
-        return s;
-    return (full_p && full_q) ? ov_find<MAP>(T, key, sk) : nullptr;
+    const Slot<VB> sp = probe_find<VB>(bucket_at<VB>(T, p), sk, c, full_p);
+    if(sp.k)
+        return sp;
+    const Slot<VB> sq = probe_find<VB>(bucket_at<VB>(T, q), sk, c, full_q);
+    if(sq.k)
+        return sq;
+    if(full_p && full_q)
+        return ov_find<VB>(T, key, sk);
+    return Slot<VB>(nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------
-// Values.  An empty slot's value is 0; every operation here is one 64-bit
-// atomic (plus a rare fix-up), written so that the winner of a key's
-// insertion and concurrent updaters of it commute.
+// Values.  An empty slot's value is 0; every operation here is one atomic on
+// the value's word (plus a rare fix-up), written so that the winner of a
+// key's insertion and concurrent updaters of it commute.  Operands are at
+// most vmax, the width's largest value.
 
-// v + x saturating at vmax (capi/Kmer_Table.hpp:38): the sum, and a clamp by
-// whoever crosses vmax.  Narrow widths never wrap their 64-bit word.
-__device__ __forceinline__ void sat_add(uint64_t* v, uint64_t x, uint64_t vmax)
+// v + x saturating at vmax (capi/Kmer_Table.hpp:38): the sum, and vmax
+// written by whoever crosses it.  Every add whose sum passes vmax (one that
+// wraps the word included) sees old > vmax - x, and its write of vmax comes
+// after its add, so once any sum has crossed, the word's last operation is a
+// write of vmax; adds that landed between a crossing and its write vanish
+// into the maximum, as they must.
+template <typename T>
+__device__ __forceinline__ void sat_add(T* v, T x, T vmax)
 {
-    const uint64_t old = add64(v, x);
-    if(old > vmax - x)
-    {
-        if(vmax == ~uint64_t(0))
-            __hip_atomic_exchange(v, vmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-            min64(v, vmax);
-    }
+    if(at_add(v, x) > vmax - x)
+        at_xchg(v, vmax);
 }
 
 // v + x modulo 2^(8 * width): whoever carries out of the width clears it.
-__device__ __forceinline__ void wrap_add(uint64_t* v, uint64_t x, uint64_t vmax)
+template <typename T>
+__device__ __forceinline__ void wrap_add(T* v, T x, T vmax)
 {
-    const uint64_t old = add64(v, x);
-    if(vmax != ~uint64_t(0) && (old & vmax) + x > vmax)
-        and64(v, vmax);
+    const T old = at_add(v, x);
+    if(vmax != ~T(0) && (old & vmax) + x > vmax)
+        at_and(v, vmax);
 }
 
 // Applies `op` to the value `v` with operand `x`; `winner` iff this call just
 // inserted the key, with the value 0 in place.
-template <uint32_t OP>
-__device__ __forceinline__ void value_op(uint64_t* v, uint64_t x, uint64_t vmax, bool winner)
+template <uint32_t OP, typename T>
+__device__ __forceinline__ void value_op(T* v, uint64_t x64, uint64_t vmax64, bool winner)
 {
+    const T x = static_cast<T>(x64), vmax = static_cast<T>(vmax64);
     if constexpr(OP == OP_KEEP)
     {
         if(winner)
-            st_rel(v, x);
+            at_store(v, x);
     }
     else if constexpr(OP == OP_ASSIGN)
-        st_rel(v, x);
+        at_store(v, x);
     else if constexpr(OP == OP_ADD)
         sat_add(v, x, vmax);
     else if constexpr(OP == OP_OR)
-        or64(v, x);
+        at_or(v, x);
     else if constexpr(OP == OP_MIN)
     {
         if(winner)
-            st_rel(v, x);
+            at_store(v, x);
         else
-            min64(v, x);
+            at_min(v, x);
     }
     else if constexpr(OP == OP_MAX)
-        max64(v, x);
+        at_max(v, x);
     else
         wrap_add(v, x, vmax);
 }

@@ -79,7 +79,7 @@ __device__ __forceinline__ void hand_off(const Handoff& h)
 
 // WAVES: the waves per SIMD its registers must allow (the occupancy shape
 // the launch tuner chooses among).
-template <bool MAP, uint32_t OP, bool ORIENT, int WAVES = KAMD_INSERT_WAVES>
+template <int VB, uint32_t OP, bool ORIENT, int WAVES = KAMD_INSERT_WAVES>
 __global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
 {
     hand_off(prev);
@@ -97,30 +97,30 @@ __global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_
     {
         const uint64_t key = active ? q.key[i] : 0, h = active ? q.h[i] : 0;
         const uint32_t cf = active ? q.cf[i] : 0;
-        uint64_t* slot;
+        Slot<VB> slot;
         bool in_ov;
-        const Res r = insert_key_wave<MAP>(T, active, key, h, static_cast<uint8_t>(cf), slot, in_ov);
+        const Res r = insert_key_wave<VB>(T, active, key, h, static_cast<uint8_t>(cf), slot, in_ov);
         // The wave's counts (uniform: scalar registers).
         ins += __popcll(__ballot(active && r == RES_NEW));
         ins_ov += __popcll(__ballot(active && r == RES_NEW && in_ov));
         failed += __popcll(__ballot(active && r == RES_FULL));
         if(active && r != RES_FULL)
-            if constexpr(MAP)
-                value_op<OP>(slot + VAL_OFF, (ORIENT && !(cf >> 8)) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
+            if constexpr(VB != 0)
+                value_op<OP>(slot.v(), (ORIENT && !(cf >> 8)) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
     };
 
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
-        uint64_t* slot = nullptr;
+        Slot<VB> slot(nullptr, 0);
         if(d.valid)
         {
             bool full;
-            slot = probe_find<MAP>(bucket_at<MAP>(T, bucket_of(d.h, T.cap)), ~d.key, d.c, full);
-            if constexpr(MAP)
-                if(slot)
-                    value_op<OP>(slot + VAL_OFF, (ORIENT && !d.fwd) ? x_rev : x, T.vmax, false);
+            slot = probe_find<VB>(bucket_at<VB>(T, bucket_of(d.h, T.cap)), ~d.key, d.c, full);
+            if constexpr(VB != 0)
+                if(slot.k)
+                    value_op<OP>(slot.v(), (ORIENT && !d.fwd) ? x_rev : x, T.vmax, false);
         }
-        const bool defer = d.valid && !slot;
+        const bool defer = d.valid && !slot.k;
         const uint64_t m = __ballot(defer);
         if(defer)
         {
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_
 }
 
 // Present-only updates (include/Streaming_Kmer_Hash_Table.hpp:2648-2691).
-template <uint32_t OP>
+template <int VB, uint32_t OP>
 __global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, Counters* cnt)
 {
     __shared__ WaveLds wave_lds[TILE / 64];
@@ -156,10 +156,10 @@ __global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restri
     for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
         if(d.valid)
-            if(uint64_t* slot = find_key<true>(T, d.key, d.h, d.c))
+            if(const Slot<VB> slot = find_key<VB>(T, d.key, d.h, d.c); slot.k)
             {
                 present++;
-                value_op<OP>(slot + VAL_OFF, x, T.vmax, false);
+                value_op<OP>(slot.v(), x, T.vmax, false);
             }
     });
     block_add(present, &cnt->present, red);
@@ -202,7 +202,7 @@ __device__ __forceinline__ uint32_t segment_of(const QueryArgs& a, uint64_t tile
     return lo;
 }
 
-template <bool MAP, int MODE>
+template <int VB, int MODE>
 __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restrict__ text, uint64_t pos_begin, uint64_t pos_end, QueryArgs a, Counters* cnt)
 {
     __shared__ WaveLds wave_lds[TILE / 64];
@@ -215,8 +215,12 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
         uint64_t v = 0;
         bool hit = false;
         if(d.valid)
-            if(const uint64_t* slot = find_key<MAP>(T, d.key, d.h, d.c))
-                hit = true, v = MAP ? slot[VAL_OFF] : 0;
+            if(const Slot<VB> slot = find_key<VB>(T, d.key, d.h, d.c); slot.k)
+            {
+                hit = true;
+                if constexpr(VB != 0)
+                    v = *slot.v();
+            }
 
         if constexpr(MODE == Q_TOTAL)
             total += hit, positions += d.valid;
@@ -279,10 +283,10 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
 // The keys away from their primary bucket are marked in `away`: bit j of
 // word w is slot j % 32 of old bucket 2w + j / 32, written by one lane per
 // wave, with no atomics.
-template <bool MAP>
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64_t* away)
 {
-    constexpr uint32_t BB = Geo<MAP>::bucket_bytes, IMG = 2 * BB;
+    constexpr uint32_t BB = Geo<VB>::bucket_bytes, IMG = 2 * BB;
     __shared__ __align__(16) uint8_t images[TILE / 64][2][IMG];
     const uint32_t lane = threadIdx.x & 63, j = lane & 31;
     const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
@@ -302,11 +306,11 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
         const uint64_t b = 2 * w + (lane >> 5);
         if(w < pairs && b < O.cap)
         {
-            const uint8_t* ob = bucket_at<MAP>(O, b);
-            const uint64_t* os = slot_at<MAP>(const_cast<uint8_t*>(ob), j);
-            o.c = ob[j], o.sk = os[0];
-            if(MAP)
-                o.val = os[VAL_OFF];
+            uint8_t* ob = bucket_at<VB>(O, b);
+            const Slot<VB> os = slot_ref<VB>(ob, j);
+            o.c = ob[j], o.sk = os.k[0];
+            if constexpr(VB != 0)
+                o.val = *os.v();
         }
         return o;
     };
@@ -347,16 +351,16 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
         {
             const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
             uint8_t* nb = img + (lo ? 0 : BB);
-            uint64_t* ns = reinterpret_cast<uint64_t*>(nb + META_BYTES) + r;
-            ns[0] = sk;
-            if(MAP)
-                ns[VAL_OFF] = val;
+            const Slot<VB> ns = slot_ref<VB>(nb, r);
+            ns.k[0] = sk;
+            if constexpr(VB != 0)
+                *ns.v() = static_cast<typename Geo<VB>::V>(val);
             nb[r] = c;
         }
         wave_sync();
         if(b < O.cap)
         {
-            uint4* dst = reinterpret_cast<uint4*>(bucket_at<MAP>(N, 2 * b));
+            uint4* dst = reinterpret_cast<uint4*>(bucket_at<VB>(N, 2 * b));
             for(uint32_t k = j; k < IMG / 16; k += 32)
                 dst[k] = reinterpret_cast<const uint4*>(img)[k];
         }
@@ -365,31 +369,15 @@ __global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64
     }
 }
 
-// Re-inserts a key with the insertion policy, its checksum and minimizer
-// recomputed (:1305-1371).
-template <bool MAP>
-__device__ __forceinline__ void reinsert_one(const TableView& N, uint64_t sk, uint64_t val, uint64_t& ov_n, uint64_t& failed)
-{
-    uint64_t* slot;
-    bool in_ov;
-    const uint64_t key = ~sk;
-    const Res r = insert_key<MAP>(N, key, minimizer_hash(key), checksum(key), slot, in_ov);
-    if(r == RES_FULL)
-        failed++;
-    else
-    {
-        ov_n += in_ov;
-        if(MAP)
-            st_rel(slot + VAL_OFF, val);
-    }
-}
-
 // Phase 2: the keys marked away from their primary buckets, then the old
 // overflow table's, go in with the insertion policy (:1575-1665).  A wave
 // takes 64 words of marks (128 old buckets), numbers their marked keys by a
 // prefix sum over the lanes, and inserts them 64 at a time, one per lane,
 // with the wave-cooperative claims; then 64 old overflow slots at a time.
-template <bool MAP>
+// A claimed slot's value is written by its claimer alone and read by no one
+// before the launch ends, so it takes a plain store (written back with the
+// launch's other stores), not a device-scope one.
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableView O, TableView N, Counters* cnt)
 {
     __shared__ uint64_t red[TILE / 64];
@@ -431,15 +419,15 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                     for(uint32_t i = k - before; i; --i)
                         m &= m - 1;
                     const uint32_t bit = __builtin_ctzll(m);
-                    const uint64_t* os = slot_at<MAP>(bucket_at<MAP>(O, 2 * (ch * 64 + src) + (bit >> 5)), bit & 31);
-                    sk = os[0];
-                    if(MAP)
-                        val = os[VAL_OFF];
+                    const Slot<VB> os = slot_ref<VB>(bucket_at<VB>(O, 2 * (ch * 64 + src) + (bit >> 5)), bit & 31);
+                    sk = os.k[0];
+                    if constexpr(VB != 0)
+                        val = *os.v();
                 }
-                uint64_t* slot;
+                Slot<VB> slot;
                 bool in_ov;
                 const uint64_t key = ~sk;
-                const Res r = insert_new_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+                const Res r = insert_new_wave<VB>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
                 if(active)
                 {
                     if(r == RES_FULL)
@@ -447,8 +435,8 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                     else
                     {
                         ov_n += in_ov;
-                        if(MAP)
-                            st_rel(slot + VAL_OFF, val);
+                        if constexpr(VB != 0)
+                            *slot.v() = static_cast<typename Geo<VB>::V>(val);
                     }
                 }
             }
@@ -459,16 +447,17 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
             uint64_t sk = 0, val = 0;
             if(i <= O.ov_mask)
             {
-                const uint64_t* os = ov_slot<MAP>(O.ov, i);
-                sk = os[0];
-                if(MAP && sk)
-                    val = os[VAL_OFF];
+                const Slot<VB> os = ov_ref<VB>(O.ov, i);
+                sk = os.k[0];
+                if constexpr(VB != 0)
+                    if(sk)
+                        val = *os.v();
             }
             const bool active = sk != 0;
-            uint64_t* slot;
+            Slot<VB> slot;
             bool in_ov;
             const uint64_t key = ~sk;
-            const Res r = insert_new_wave<MAP>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+            const Res r = insert_new_wave<VB>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
             if(active)
             {
                 if(r == RES_FULL)
@@ -476,8 +465,8 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
                 else
                 {
                     ov_n += in_ov;
-                    if(MAP)
-                        st_rel(slot + VAL_OFF, val);
+                    if constexpr(VB != 0)
+                        *slot.v() = static_cast<typename Geo<VB>::V>(val);
                 }
             }
         }
@@ -488,7 +477,7 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
 
 // Overflow growth: the old overflow table's keys into a new one
 // (include/Streaming_Kmer_Hash_Table.hpp:1412-1443).
-template <bool MAP>
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint64_t old_slots, TableView N, Counters* cnt)
 {
     __shared__ uint64_t red[TILE / 64];
@@ -496,15 +485,15 @@ __global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < old_slots; g += stride)
     {
-        const uint64_t* os = ov_slot<MAP>(const_cast<uint64_t*>(old_ov), g);
-        const uint64_t sk = os[0];
+        const Slot<VB> os = ov_ref<VB>(const_cast<uint64_t*>(old_ov), g);
+        const uint64_t sk = os.k[0];
         if(!sk)
             continue;
-        uint64_t* slot;
-        if(ov_insert<MAP>(N, ~sk, sk, slot) == RES_FULL)
+        Slot<VB> slot;
+        if(ov_insert<VB>(N, ~sk, sk, slot) == RES_FULL)
             failed++;
-        else if(MAP)
-            st_rel(slot + VAL_OFF, os[VAL_OFF]);
+        else if constexpr(VB != 0)
+            at_store(slot.v(), *os.v());
     }
     block_add(failed, &cnt->failed, red);
 }
@@ -514,7 +503,7 @@ __global__ __launch_bounds__(TILE) void k_ov_rehash(const uint64_t* old_ov, uint
 
 // Copies the keys (2-bit encodings) and values of buckets [b0, b1) and
 // overflow slots [o0, o1) to out_keys / out_vals, at cursor positions.
-template <bool MAP>
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint64_t b1, uint64_t o0, uint64_t o1, uint64_t* out_keys, uint64_t* out_vals, uint64_t* cursor)
 {
     const uint64_t main_n = (b1 - b0) * B, all = main_n + (o1 - o0);
@@ -526,23 +515,24 @@ __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint
         uint64_t sk = 0, val = 0;
         if(g < main_n)
         {
-            uint8_t* bk = bucket_at<MAP>(T, b0 + g / B);
+            uint8_t* bk = bucket_at<VB>(T, b0 + g / B);
             if(bk[g % B])
             {
-                const uint64_t* s = slot_at<MAP>(bk, g % B);
-                sk = s[0];
+                const Slot<VB> s = slot_ref<VB>(bk, g % B);
+                sk = s.k[0];
                 if(!is_key(sk))
                     sk = 0;
-                else if(MAP)
-                    val = s[VAL_OFF];
+                else if constexpr(VB != 0)
+                    val = *s.v();
             }
         }
         else if(g < all)
         {
-            const uint64_t* s = ov_slot<MAP>(T.ov, o0 + g - main_n);
-            sk = s[0];
-            if(MAP && sk)
-                val = s[VAL_OFF];
+            const Slot<VB> s = ov_ref<VB>(T.ov, o0 + g - main_n);
+            sk = s.k[0];
+            if constexpr(VB != 0)
+                if(sk)
+                    val = *s.v();
         }
         const uint64_t m = __ballot(sk != 0);
         if(!m)
@@ -556,7 +546,7 @@ __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint
         {
             const uint64_t i = base + __popcll(m & ((uint64_t(1) << lane) - 1));
             out_keys[i] = ~sk;
-            if(MAP)
+            if constexpr(VB != 0)
                 out_vals[i] = val;
         }
     }
@@ -564,7 +554,7 @@ __global__ __launch_bounds__(TILE) void k_collect(TableView T, uint64_t b0, uint
 
 // capi/Kmer_Table.hpp:203-230: KEYS (v & m nonzero; every key of a set),
 // POPCOUNT, SUM of v & m.
-template <bool MAP>
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uint64_t mask, uint64_t* out)
 {
     __shared__ uint64_t red[TILE / 64];
@@ -573,28 +563,28 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
     uint64_t acc = 0;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
     {
-        const uint64_t* s;
+        Slot<VB> s;
         if(g < main_n)
         {
-            uint8_t* bk = bucket_at<MAP>(T, g / B);
+            uint8_t* bk = bucket_at<VB>(T, g / B);
             if(!bk[g % B])
                 continue;
-            s = slot_at<MAP>(bk, g % B);
-            if(!is_key(s[0]))
+            s = slot_ref<VB>(bk, g % B);
+            if(!is_key(s.k[0]))
                 continue;
         }
         else
         {
-            s = ov_slot<MAP>(T.ov, g - main_n);
-            if(!s[0])
+            s = ov_ref<VB>(T.ov, g - main_n);
+            if(!s.k[0])
                 continue;
         }
-        if(!MAP)
+        if constexpr(VB == 0)
         {
             acc++;
             continue;
         }
-        const uint64_t v = s[VAL_OFF] & mask;
+        const uint64_t v = *s.v() & mask;
         acc += what == 0 ? (v != 0) : what == 1 ? static_cast<uint64_t>(__popcll(v)) : v;
     }
     block_add(acc, out, red);
@@ -604,7 +594,7 @@ __global__ __launch_bounds__(TILE) void k_reduce(TableView T, uint32_t what, uin
 // in [1, max] (counter/Kmer_Set_Counter.cpp:97-120).  LDS bins per block
 // when they fit (dynamic shared memory of (max + 1) words), else global
 // atomics.
-template <bool LDS>
+template <int VB, bool LDS>
 __global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, uint64_t* out)
 {
     extern __shared__ uint32_t bins[];
@@ -618,23 +608,23 @@ __global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, u
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
     {
-        const uint64_t* s;
+        Slot<VB> s;
         if(g < main_n)
         {
-            uint8_t* bk = bucket_at<true>(T, g / B);
+            uint8_t* bk = bucket_at<VB>(T, g / B);
             if(!bk[g % B])
                 continue;
-            s = slot_at<true>(bk, g % B);
-            if(!is_key(s[0]))
+            s = slot_ref<VB>(bk, g % B);
+            if(!is_key(s.k[0]))
                 continue;
         }
         else
         {
-            s = ov_slot<true>(T.ov, g - main_n);
-            if(!s[0])
+            s = ov_ref<VB>(T.ov, g - main_n);
+            if(!s.k[0])
                 continue;
         }
-        const uint64_t v = s[VAL_OFF];
+        const uint64_t v = *s.v();
         if(v < 1 || v > max)
             continue;
         if constexpr(LDS)
@@ -653,7 +643,7 @@ __global__ __launch_bounds__(TILE) void k_histogram(TableView T, uint64_t max, u
 
 // Keys of the main table at their primary bucket, and elsewhere
 // (include/Streaming_Kmer_Hash_Table.hpp:1109-1120, 2728-2752).
-template <bool MAP>
+template <int VB>
 __global__ __launch_bounds__(TILE) void k_placement(TableView T, uint64_t* out)
 {
     __shared__ uint64_t red[TILE / 64];
@@ -661,10 +651,10 @@ __global__ __launch_bounds__(TILE) void k_placement(TableView T, uint64_t* out)
     uint64_t prim = 0, sec = 0;
     for(uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < all; g += stride)
     {
-        uint8_t* bk = bucket_at<MAP>(T, g / B);
-        if(!bk[g % B] || !is_key(slot_at<MAP>(bk, g % B)[0]))
+        uint8_t* bk = bucket_at<VB>(T, g / B);
+        if(!bk[g % B] || !is_key(slot_at<VB>(bk, g % B)[0]))
             continue;
-        const uint64_t h = minimizer_hash(~slot_at<MAP>(bk, g % B)[0]);
+        const uint64_t h = minimizer_hash(~slot_at<VB>(bk, g % B)[0]);
         if(bucket_of(h, T.cap) == g / B)
             prim++;
         else

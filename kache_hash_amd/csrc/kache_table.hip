@@ -494,6 +494,7 @@ struct kache_table
 
     const double lf;
     const bool map;
+    const uint32_t vw;                      // value bytes per slot: 0 (a set), 4 (widths 1-4), 8 (widths 5-8)
     const int device;                       // the device current at creation: the table's arrays live there
     DeviceScratch* dscratch = nullptr;      // that device's scratch (allocation-free calls)
 
@@ -536,7 +537,7 @@ struct kache_table
     mutable size_t ev_used = 0;
 
     kache_table(const kache_options& o)
-        : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0), device(current_device())
+        : value_bytes(o.value_bytes), lf(o.load_factor), map(o.value_bytes > 0), vw(value_word(o.value_bytes)), device(current_device())
     {
         HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
@@ -613,11 +614,26 @@ struct kache_table
             (void)hipStreamDestroy(stream), stream = nullptr;
     }
 
-    uint32_t bucket_bytes() const { return map ? Geo<true>::bucket_bytes : Geo<false>::bucket_bytes; }
-    uint32_t slot_words() const { return map ? 2 : 1; }
+    static uint32_t bucket_bytes_of(uint32_t vw) { return vw == 0 ? Geo<0>::bucket_bytes : vw == 4 ? Geo<4>::bucket_bytes : Geo<8>::bucket_bytes; }
+    uint32_t bucket_bytes() const { return bucket_bytes_of(vw); }
     uint64_t vmax() const { return value_bytes == 8 ? ~uint64_t(0) : value_bytes == 0 ? 0 : (uint64_t(1) << (8 * value_bytes)) - 1; }
 
-    uint64_t ov_bytes(uint64_t slots) const { return slots * slot_words() * 8; }
+    // Overflow slots come in blocks of 32 (at least OV_MIN of them).
+    uint64_t ov_bytes(uint64_t slots) const { return slots * (8 + vw); }
+
+    // Calls f with the table's value bytes per slot as a compile-time
+    // constant (std::integral_constant<int, 0, 4 or 8>): the kernels'
+    // layout parameter.
+    template <typename F>
+    decltype(auto) by_layout(F&& f) const
+    {
+        switch(vw)
+        {
+            case 0: return f(std::integral_constant<int, 0>{});
+            case 4: return f(std::integral_constant<int, 4>{});
+            default: return f(std::integral_constant<int, 8>{});
+        }
+    }
 
     // Bucket and overflow arrays come from the array cache and go back to it.
     uint8_t* take_buckets(uint64_t n) const { return static_cast<uint8_t*>(ArrayCache::get().take(device, n * bucket_bytes())); }
@@ -744,17 +760,17 @@ struct kache_table
 
     void rehash_ov(uint64_t new_slots)
     {
-        check_budget(new_slots * slot_words() * 8, "growing the overflow table");
+        check_budget(ov_bytes(new_slots), "growing the overflow table");
         uint64_t* nov = alloc_ov(new_slots);
         try
         {
             const TableView N = view_of(buckets, cap, nov, new_slots);
             const uint64_t blocks = (ov_slots + TILE - 1) / TILE;
             timed("ov_rehash", [&]{
-                if(map)
-                    hipLaunchKernelGGL(k_ov_rehash<true>, dim3(grid_for(k_ov_rehash<true>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
-                else
-                    hipLaunchKernelGGL(k_ov_rehash<false>, dim3(grid_for(k_ov_rehash<false>, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+                by_layout([&](auto L){
+                    const auto kern = k_ov_rehash<decltype(L)::value>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, blocks)), dim3(TILE), 0, stream, ov, ov_slots, N, d_cnt);
+                });
             });
             const Counters c = read_counters();
             if(c.failed)
@@ -777,7 +793,7 @@ struct kache_table
     // during growth included (0: no limit but the device's own).
     uint64_t hbm_budget = 0;
 
-    uint64_t device_bytes() const { return cap * bucket_bytes() + ov_slots * slot_words() * 8; }
+    uint64_t device_bytes() const { return cap * bucket_bytes() + ov_bytes(ov_slots); }
 
     // HBM the next doubling needs beside the current arrays: the doubled
     // bucket store, its overflow table and the split's bitmap.
@@ -964,16 +980,16 @@ struct kache_table
             const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
             const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
             timed("split", [&]{
-                if(map)
-                    hipLaunchKernelGGL(k_split<true>, dim3(grid_for(k_split<true>, split_blocks)), dim3(TILE), 0, stream, O, N, away);
-                else
-                    hipLaunchKernelGGL(k_split<false>, dim3(grid_for(k_split<false>, split_blocks)), dim3(TILE), 0, stream, O, N, away);
+                by_layout([&](auto L){
+                    const auto kern = k_split<decltype(L)::value>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, split_blocks)), dim3(TILE), 0, stream, O, N, away);
+                });
             });
             timed("reinsert", [&]{
-                if(map)
-                    hipLaunchKernelGGL(k_reinsert<true>, dim3(grid_for(k_reinsert<true>, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
-                else
-                    hipLaunchKernelGGL(k_reinsert<false>, dim3(grid_for(k_reinsert<false>, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
+                by_layout([&](auto L){
+                    const auto kern = k_reinsert<decltype(L)::value>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
+                });
             });
             c = read_counters();
             if(c.failed)
@@ -1167,30 +1183,36 @@ struct kache_table
     // --- Inserts ---
 
     // The insert kernel of an operation, at `waves` waves per SIMD.
-    template <int WV>
-    const void* insert_kernel_at(uint32_t op, bool oriented) const
+    template <int VB, int WV>
+    static const void* insert_kernel_of(uint32_t op, bool oriented)
     {
         const auto f = [](auto kern) { return reinterpret_cast<const void*>(kern); };
-        if(!map)
-            return f(k_insert<false, OP_KEEP, false, WV>);
         if(oriented)
             switch(op)
             {
-            case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, true, WV>);
-            case OP_ADD:    return f(k_insert<true, OP_ADD, true, WV>);
-            case OP_OR:     return f(k_insert<true, OP_OR, true, WV>);
-            default:        return f(k_insert<true, OP_MAX, true, WV>);
+            case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, true, WV>);
+            case OP_ADD:    return f(k_insert<VB, OP_ADD, true, WV>);
+            case OP_OR:     return f(k_insert<VB, OP_OR, true, WV>);
+            default:        return f(k_insert<VB, OP_MAX, true, WV>);
             }
         switch(op)
         {
-        case OP_KEEP:   return f(k_insert<true, OP_KEEP, false, WV>);
-        case OP_ASSIGN: return f(k_insert<true, OP_ASSIGN, false, WV>);
-        case OP_ADD:    return f(k_insert<true, OP_ADD, false, WV>);
-        case OP_OR:     return f(k_insert<true, OP_OR, false, WV>);
-        case OP_MIN:    return f(k_insert<true, OP_MIN, false, WV>);
-        case OP_MAX:    return f(k_insert<true, OP_MAX, false, WV>);
-        default:        return f(k_insert<true, OP_ADD_WRAP, false, WV>);
+        case OP_KEEP:   return f(k_insert<VB, OP_KEEP, false, WV>);
+        case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, false, WV>);
+        case OP_ADD:    return f(k_insert<VB, OP_ADD, false, WV>);
+        case OP_OR:     return f(k_insert<VB, OP_OR, false, WV>);
+        case OP_MIN:    return f(k_insert<VB, OP_MIN, false, WV>);
+        case OP_MAX:    return f(k_insert<VB, OP_MAX, false, WV>);
+        default:        return f(k_insert<VB, OP_ADD_WRAP, false, WV>);
         }
+    }
+
+    template <int WV>
+    const void* insert_kernel_at(uint32_t op, bool oriented) const
+    {
+        if(!map)
+            return reinterpret_cast<const void*>(k_insert<0, OP_KEEP, false, WV>);
+        return vw == 4 ? insert_kernel_of<4, WV>(op, oriented) : insert_kernel_of<8, WV>(op, oriented);
     }
 
     const void* insert_kernel(uint32_t op, bool oriented, uint32_t waves = 0) const
@@ -1818,10 +1840,10 @@ struct kache_table
             const uint64_t tiles = (e - (b & ~uint64_t(15)) + TILE - 1) / TILE;
             Counters* c = cnt.as<Counters>() + i;
             timed("query", [&]{
-                if(map)
-                    hipLaunchKernelGGL((k_query<true, Q_TOTAL>), dim3(grid_for(k_query<true, Q_TOTAL>, tiles)), dim3(TILE), 0, stream, view(), text, b, e, a, c);
-                else
-                    hipLaunchKernelGGL((k_query<false, Q_TOTAL>), dim3(grid_for(k_query<false, Q_TOTAL>, tiles)), dim3(TILE), 0, stream, view(), text, b, e, a, c);
+                by_layout([&](auto L){
+                    const auto kern = k_query<decltype(L)::value, Q_TOTAL>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, tiles)), dim3(TILE), 0, stream, view(), text, b, e, a, c);
+                });
             });
         }
         std::vector<Counters> h(n);
@@ -1863,7 +1885,8 @@ struct kache_table
     void launch_update(const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x) const
     {
         const uint64_t tiles = (e - b + TILE - 1) / TILE;
-        hipLaunchKernelGGL(k_update<OP>, dim3(grid_for(k_update<OP>, tiles)), dim3(TILE), 0, stream, view(), text, text_len, b, e, x, d_cnt);
+        const auto kern = vw == 4 ? k_update<4, OP> : k_update<8, OP>;
+        hipLaunchKernelGGL(kern, dim3(grid_for(kern, tiles)), dim3(TILE), 0, stream, view(), text, text_len, b, e, x, d_cnt);
     }
 
     uint64_t update_text(uint32_t op, const uint8_t* text, uint64_t text_len, uint64_t x) const
@@ -1912,10 +1935,10 @@ struct kache_table
     {
         const uint64_t tiles = (text_len + TILE - 1) / TILE;
         timed("query", [&]{
-            if(map)
-                hipLaunchKernelGGL((k_query<true, MODE>), dim3(grid_for(k_query<true, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
-            else
-                hipLaunchKernelGGL((k_query<false, MODE>), dim3(grid_for(k_query<false, MODE>, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
+            by_layout([&](auto L){
+                const auto kern = k_query<decltype(L)::value, MODE>;
+                hipLaunchKernelGGL(kern, dim3(grid_for(kern, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
+            });
         });
     }
 
@@ -2168,12 +2191,10 @@ struct kache_table
         HIP(hipGetLastError());
         QueryArgs a{};
         a.seg_start = seg_start, a.seg_q = seg_q, a.tile_seg = tile_seg, a.per_seq = per_seq;
-        if(map)
-            hipLaunchKernelGGL((k_query<true, Q_COUNT>), dim3(grid_for(k_query<true, Q_COUNT>, tiles)), dim3(TILE), 0, stream, view(), text,
-                               uint64_t(0), text_len, a, d_cnt);
-        else
-            hipLaunchKernelGGL((k_query<false, Q_COUNT>), dim3(grid_for(k_query<false, Q_COUNT>, tiles)), dim3(TILE), 0, stream, view(), text,
-                               uint64_t(0), text_len, a, d_cnt);
+        by_layout([&](auto L){
+            const auto kern = k_query<decltype(L)::value, Q_COUNT>;
+            hipLaunchKernelGGL(kern, dim3(grid_for(kern, tiles)), dim3(TILE), 0, stream, view(), text, uint64_t(0), text_len, a, d_cnt);
+        });
         HIP(hipGetLastError());
         HIP(hipMemcpyAsync(out, per_seq, n * 8, hipMemcpyDeviceToHost, stream));
         HIP(hipStreamSynchronize(stream));
@@ -2239,10 +2260,11 @@ struct kache_table
             const uint64_t work = (pc.b1 - pc.b0) * B + (pc.o1 - pc.o0);
             const uint64_t blocks = (work + TILE - 1) / TILE;
             timed("collect", [&]{
-                if(map)
-                    hipLaunchKernelGGL(k_collect<true>, dim3(grid_for(k_collect<true>, blocks)), dim3(TILE), 0, stream, view(), pc.b0, pc.b1, pc.o0, pc.o1, dk.as<uint64_t>(), dv.as<uint64_t>(), dc.as<uint64_t>());
-                else
-                    hipLaunchKernelGGL(k_collect<false>, dim3(grid_for(k_collect<false>, blocks)), dim3(TILE), 0, stream, view(), pc.b0, pc.b1, pc.o0, pc.o1, dk.as<uint64_t>(), static_cast<uint64_t*>(nullptr), dc.as<uint64_t>());
+                by_layout([&](auto L){
+                    const auto kern = k_collect<decltype(L)::value>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, blocks)), dim3(TILE), 0, stream, view(), pc.b0, pc.b1, pc.o0, pc.o1, dk.as<uint64_t>(),
+                                       map ? dv.as<uint64_t>() : static_cast<uint64_t*>(nullptr), dc.as<uint64_t>());
+                });
             });
             uint64_t n = 0;
             HIP(hipMemcpyAsync(&n, dc.p, 8, hipMemcpyDeviceToHost, stream));
@@ -2269,10 +2291,10 @@ struct kache_table
         HIP(hipMemsetAsync(out.p, 0, 8, stream));
         const uint64_t blocks = (cap * B + ov_slots + TILE - 1) / TILE;
         timed("reduce", [&]{
-            if(map)
-                hipLaunchKernelGGL(k_reduce<true>, dim3(grid_for(k_reduce<true>, blocks)), dim3(TILE), 0, stream, view(), what, mask, out.as<uint64_t>());
-            else
-                hipLaunchKernelGGL(k_reduce<false>, dim3(grid_for(k_reduce<false>, blocks)), dim3(TILE), 0, stream, view(), what, mask, out.as<uint64_t>());
+            by_layout([&](auto L){
+                const auto kern = k_reduce<decltype(L)::value>;
+                hipLaunchKernelGGL(kern, dim3(grid_for(kern, blocks)), dim3(TILE), 0, stream, view(), what, mask, out.as<uint64_t>());
+            });
         });
         uint64_t r = 0;
         HIP(hipMemcpyAsync(&r, out.p, 8, hipMemcpyDeviceToHost, stream));
@@ -2293,14 +2315,18 @@ struct kache_table
         timed("histogram", [&]{
             if(lds <= 48 * 1024)
             {
+                const auto kern = vw == 4 ? k_histogram<4, true> : k_histogram<8, true>;
                 int per_cu = 0;
-                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_histogram<true>, TILE, lds) != hipSuccess || per_cu < 1)
+                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TILE, lds) != hipSuccess || per_cu < 1)
                     per_cu = 1;
                 const uint64_t g = std::min<uint64_t>(blocks, uint64_t(per_cu) * device_cus());
-                hipLaunchKernelGGL(k_histogram<true>, dim3(std::max<uint64_t>(g, 1)), dim3(TILE), lds, stream, view(), max, d.as<uint64_t>());
+                hipLaunchKernelGGL(kern, dim3(std::max<uint64_t>(g, 1)), dim3(TILE), lds, stream, view(), max, d.as<uint64_t>());
             }
             else
-                hipLaunchKernelGGL(k_histogram<false>, dim3(grid_for(k_histogram<false>, blocks)), dim3(TILE), 0, stream, view(), max, d.as<uint64_t>());
+            {
+                const auto kern = vw == 4 ? k_histogram<4, false> : k_histogram<8, false>;
+                hipLaunchKernelGGL(kern, dim3(grid_for(kern, blocks)), dim3(TILE), 0, stream, view(), max, d.as<uint64_t>());
+            }
         });
         HIP(hipMemcpyAsync(out, d.p, (max + 1) * 8, hipMemcpyDeviceToHost, stream));
         sync();
@@ -2323,6 +2349,8 @@ struct kache_table
         uint64_t header_hash;       // of the bytes before it
     };
     static constexpr char MAGIC[8] = {'K', 'A', 'M', 'D', 'T', 'B', 'L', '1'};
+    // Version 3: values of widths 1-4 in 4-byte words.
+    static constexpr uint32_t FILE_VERSION = 3;
     static constexpr uint64_t IO_CHUNK = uint64_t(64) << 20;
 
     // A word-wise hash of file bytes (a multiply-rotate mix per 8 bytes).
@@ -2341,7 +2369,7 @@ struct kache_table
     // The device regions of the image, in file order.
     std::vector<std::pair<uint8_t*, uint64_t>> regions() const
     {
-        return {{buckets, cap * bucket_bytes()}, {reinterpret_cast<uint8_t*>(ov), ov_slots * slot_words() * 8}};
+        return {{buckets, cap * bucket_bytes()}, {reinterpret_cast<uint8_t*>(ov), ov_bytes(ov_slots)}};
     }
 
     void save(const char* path) const
@@ -2350,7 +2378,7 @@ struct kache_table
         sync();
         FileHeader h{};
         std::memcpy(h.magic, MAGIC, 8);
-        h.version = 2, h.value_bytes = value_bytes, h.lf = lf;
+        h.version = FILE_VERSION, h.value_bytes = value_bytes, h.lf = lf;
         h.cap = cap, h.ov_slots = ov_slots, h.ov_count = ov_count, h.size = size, h.resizes = resizes;
         h.approx_sz = approx_sz, h.resize_th = resize_th, h.step = step;
         h.user_c = user_c.load();
@@ -2388,15 +2416,15 @@ struct kache_table
     static FileHeader read_header(FILE* f)
     {
         FileHeader h{};
-        if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, MAGIC, 8) != 0 || h.version != 2)
+        if(std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, MAGIC, 8) != 0 || h.version != FILE_VERSION)
             throw std::invalid_argument("not a table file of this version");
         if(h.header_hash != hash_words(0x6B616368655F6864ull, &h, offsetof(FileHeader, header_hash)))
             throw std::invalid_argument("the table file's header is corrupt");
         const uint32_t vb = h.value_bytes;
-        const uint64_t bb = vb ? Geo<true>::bucket_bytes : Geo<false>::bucket_bytes;
+        const uint32_t w = value_word(vb);
         if(!(vb == 0 || vb == 1 || vb == 2 || vb == 4 || vb == 8) || !(h.lf > 0 && h.lf < 1) || h.cap == 0 || h.cap > MAX_CAP ||
-           h.ov_slots == 0 || (h.ov_slots & (h.ov_slots - 1)) || h.user_c > KACHE_MAX_USERS ||
-           h.data_bytes != h.cap * bb + h.ov_slots * (vb ? 2 : 1) * 8)
+           h.ov_slots < OV_MIN || (h.ov_slots & (h.ov_slots - 1)) || h.user_c > KACHE_MAX_USERS ||
+           h.data_bytes != h.cap * bucket_bytes_of(w) + h.ov_slots * (8 + w))
             throw std::invalid_argument("the table file's header is inconsistent");
         return h;
     }
@@ -2461,10 +2489,10 @@ struct kache_table
         out.reserve(16);
         HIP(hipMemsetAsync(out.p, 0, 16, stream));
         const uint64_t blocks = (cap * B + TILE - 1) / TILE;
-        if(map)
-            hipLaunchKernelGGL(k_placement<true>, dim3(grid_for(k_placement<true>, blocks)), dim3(TILE), 0, stream, view(), out.as<uint64_t>());
-        else
-            hipLaunchKernelGGL(k_placement<false>, dim3(grid_for(k_placement<false>, blocks)), dim3(TILE), 0, stream, view(), out.as<uint64_t>());
+        by_layout([&](auto L){
+            const auto kern = k_placement<decltype(L)::value>;
+            hipLaunchKernelGGL(kern, dim3(grid_for(kern, blocks)), dim3(TILE), 0, stream, view(), out.as<uint64_t>());
+        });
         HIP(hipGetLastError());
         uint64_t ps[2] = {0, 0};
         HIP(hipMemcpyAsync(ps, out.p, 16, hipMemcpyDeviceToHost, stream));
