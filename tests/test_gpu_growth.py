@@ -72,3 +72,24 @@ def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
     assert t.stats()["capacity"] >= 2 * cap0 and t.info()["resizes"] >= 1
     same_as_oracle(t, ot, queries)
     t.close()
+
+
+@pytest.mark.parametrize("width", [0, 4, 8])
+def test_table_larger_than_the_device_is_enomem(api, width):
+    """kache_new of a table within the reference's limit of 2^32 - 1 buckets
+    (include/Streaming_Kmer_Hash_Table.hpp:84) but beyond the device's memory
+    (100G keys: 3.9G buckets, 1.3-2.3 TB) returns KACHE_ENOMEM and leaves the
+    handle NULL; the next table works."""
+    import ctypes as C
+    L = api.lib
+    o = L.kache_default_options(31, width)
+    o.expected_keys = 100_000_000_000
+    h = C.c_void_p(1234)
+    assert L.kache_new(C.byref(o), C.byref(h)) == capi.KACHE_ENOMEM and not h.value
+    g = G.genome(20_000, 91 + width)
+    with capi.Table(api, width, expected_keys=30_000) as t:
+        u = t.register()
+        ot = O.OTable(width)
+        reads = G.reads(g, 500, 150, 92)
+        assert t.insert(reads, capi.ADD if width else capi.KEEP, 1, u) == ot.insert(reads, capi.ADD if width else capi.KEEP, 1)
+        same_as_oracle(t, ot, reads[:50])
