@@ -1,0 +1,144 @@
+"""Seeded random programs of C-ABI calls, run through a table of any C-ABI
+library (the device library, the reference library built from its sources)
+and the oracle, with equality asserted at every step.
+
+Each case draws a width, a sizing (from a tiny table that doubles many times
+to an exact one, plain or power-of-2 capacity), one to three users, and a
+program of calls over random sequences: pieces of a genome in either strand,
+lengths 0-600, runs of N, mixed case, tandem repeats (the same k-mer many
+times in one call) and minimizer groups (keys that overflow their buckets
+into the secondaries and the overflow table).  Calls: inserts with every
+operation and random operands (saturating ones included), oriented inserts,
+updates, lookups, counts, sums, masked counts, reductions, get, size.  Every
+return value, every query output, the capacity after every insert (against
+the reference's growth schedule, include/Streaming_Kmer_Hash_Table.hpp:
+224-237, 2246-2286) and the final contents must equal the oracle's.
+Oriented ASSIGN is left out: within one call its result may be either
+operand (test/capi.cpp:550-564)."""
+import numpy as np
+
+import oracle as O
+import seqgen as G
+from kache_hash_amd import capi
+
+CASES = 128
+KEEP, ASSIGN, ADD, OR, MIN, MAX = range(6)
+
+
+def _noise(rng, s):
+    """Runs of N and lowercase stretches over a sequence."""
+    b = bytearray(s)
+    for _ in range(int(rng.integers(0, 3))):
+        if len(b) > 2:
+            i = int(rng.integers(0, len(b)))
+            j = min(len(b), i + int(rng.integers(1, 4)))
+            b[i:j] = b"N" * (j - i)
+    if len(b) and rng.random() < 0.3:
+        i = int(rng.integers(0, len(b)))
+        j = min(len(b), i + int(rng.integers(1, 80)))
+        b[i:j] = bytes(b[i:j]).lower()
+    return bytes(b)
+
+
+def sequences(rng, genome, groups, n):
+    out = []
+    for _ in range(n):
+        r = rng.random()
+        if r < 0.08:
+            out.append(b"" if rng.random() < 0.5 else b"ACGTN"[: int(rng.integers(0, 5))] * int(rng.integers(1, 6)))
+        elif r < 0.16:
+            unit = genome[int(rng.integers(0, len(genome) - 50)):][: int(rng.integers(20, 45))]
+            out.append(unit * int(rng.integers(3, 40)))
+        elif r < 0.24 and groups:
+            g = groups[int(rng.integers(0, len(groups)))]
+            x = g[int(rng.integers(0, len(g)))]
+            out.append(G.revcomp(x) if rng.random() < 0.5 else x)
+        else:
+            ln = int(rng.integers(1, 600))
+            a = int(rng.integers(0, max(1, len(genome) - ln)))
+            s = genome[a:a + ln]
+            if rng.random() < 0.5:
+                s = G.revcomp(s)
+            out.append(_noise(rng, s))
+    return out
+
+
+def operand(rng, op, vmax):
+    if op == ADD:
+        return int(rng.choice([1, 1 + int(rng.integers(0, 7)), vmax // 2 + 1, vmax]))
+    if op == OR:
+        return 1 << int(rng.integers(0, max(1, vmax.bit_length())))
+    return int(rng.integers(0, vmax, endpoint=True, dtype=np.uint64))
+
+
+def same_contents(t, o):
+    k, v, _ = t.for_each()
+    ko, vo = o.dump()
+    assert np.array_equal(k, ko)
+    if t.width:
+        assert np.array_equal(v.astype(np.uint64), vo.astype(np.uint64))
+
+
+def run_program(api, case):
+    """Runs case `case` through a table of the library `api` (a capi.Api)
+    and the oracle, asserting equality at every step."""
+    rng = np.random.default_rng(0xC0FFEE + case)
+    width = int(rng.choice([0, 1, 2, 4, 8]))
+    vmax = (1 << (8 * width)) - 1 if width else 0
+    expected = int(rng.choice([10, 300, 5_000, 40_000]))
+    pow2 = bool(rng.random() < 0.3)
+    users = int(rng.integers(1, 4))
+    genome = G.genome(int(rng.integers(3_000, 40_000)), 7000 + case)
+    groups = G.minimizer_groups(3, 120, 8000 + case) if rng.random() < 0.5 else []
+    calls = int(rng.integers(6, 14))
+    print(f"random case {case}: width {width}, expected {expected}, pow2 {pow2}, users {users}, "
+          f"genome {len(genome)}, groups {bool(groups)}, calls {calls}", flush=True)
+    with capi.Table(api, width, expected, 0.8, pow2, 2) as t:
+        o = O.OTable(width)
+        sched = O.Sched(expected, 0.8, pow2)
+        us = [t.register() for _ in range(users)]
+        for step in range(calls):
+            seqs = sequences(rng, genome, groups, int(rng.integers(1, 60)))
+            kind = rng.random()
+            u = us[int(rng.integers(0, users))]
+            if kind < 0.45 or (width == 0 and kind < 0.7):
+                op = int(rng.integers(0, 6)) if width else KEEP
+                x = operand(rng, op, vmax) if width else 0
+                got = t.insert(seqs, op, x, u)
+                want = o.insert(seqs, op, x)
+                sched.call(u, want)
+                assert got == want, (step, "insert", op, x)
+                assert t.stats()["capacity"] == sched.cap * 32, (step, "capacity")
+            elif kind < 0.6 and width:
+                op = int(rng.choice([ADD, OR, MAX]))
+                xf, xr = operand(rng, op, vmax), operand(rng, op, vmax)
+                got = t.insert_oriented(seqs, op, xf, xr, u)
+                want = o.insert_oriented(seqs, op, xf, xr)
+                sched.call(u, want)
+                assert got == want, (step, "oriented", op, xf, xr)
+                assert t.stats()["capacity"] == sched.cap * 32, (step, "capacity")
+            elif kind < 0.7 and width:
+                op = int(rng.integers(0, 6))
+                x = operand(rng, op, vmax)
+                assert t.update(seqs, op, x) == o.update(seqs, op, x), (step, "update", op, x)
+            else:
+                f, v, fw = t.lookup(seqs)
+                f2, v2, fw2 = o.lookup(seqs)
+                assert np.array_equal(f, f2) and np.array_equal(fw, fw2), (step, "lookup")
+                if width:
+                    assert np.array_equal(v.astype(np.uint64), v2.astype(np.uint64)), (step, "lookup values")
+                c, sm, m = o.counts(seqs, 0x5555555555555555 & vmax, 0xA3A3A3A3A3A3A3A3 & vmax)
+                assert np.array_equal(t.count(seqs), c), (step, "count")
+                if width:
+                    assert np.array_equal(t.sum(seqs), sm), (step, "sum")
+                    assert np.array_equal(t.count_masked(seqs, 0x5555555555555555, 0xA3A3A3A3A3A3A3A3), m), (step, "masked")
+            assert t.size() == o.size(), (step, "size")
+        same_contents(t, o)
+        for what in ((0, 1, 2) if width else (0,)):
+            for mask in ((vmax, 0x5A5A5A5A5A5A5A5A & vmax) if width else ((1 << 64) - 1,)):
+                assert t.reduce(what, mask) == o.reduce(what, mask), ("reduce", what, mask)
+        for s in sequences(rng, genome, groups, 20):
+            if len(s) >= 31:
+                valid, key, _, _, _ = O.describe(s[:31])
+                want = o.get(int(key[0])) if valid[0] else (False, 0)
+                assert t.get(s[:31]) == want, ("get", s[:31])

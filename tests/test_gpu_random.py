@@ -1,0 +1,19 @@
+"""The seeded random programs of tests/programs.py through the device
+library, against the oracle (the same programs run against the reference
+library in test_oracle.py)."""
+import pytest
+
+import kache_hash_amd as KA
+import programs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    return KA.api()
+
+
+@pytest.mark.parametrize("case", range(programs.CASES))
+def test_random_program(api, case):
+    programs.run_program(api, case)

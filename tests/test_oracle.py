@@ -244,3 +244,16 @@ def test_oracle_matches_reference_library_live():
     assert a["present"] == b["present"] and (a["values2"] == b["values2"]).all()
     assert a["reduce"] == b["reduce"]
     assert (a["get"][0] == b["get"][0]).all() and (a["get"][1] == b["get"][1]).all()
+
+
+def test_random_programs_reference_vs_oracle():
+    """The seeded random programs (tests/programs.py) through the reference
+    library, against the oracle: every call's result, capacity and query."""
+    path = O.ref_library()
+    if path is None:
+        pytest.skip("reference library not built (make -C oracle ref)")
+    import programs
+    from kache_hash_amd import capi
+    api = capi.Api(path)
+    for case in range(programs.CASES):
+        programs.run_program(api, case)
