@@ -22,6 +22,7 @@ import seqgen as G
 from kache_hash_amd import capi
 
 CASES = 128
+DEVICE_CASES = 64
 KEEP, ASSIGN, ADD, OR, MIN, MAX = range(6)
 
 
@@ -142,3 +143,90 @@ def run_program(api, case):
                 valid, key, _, _, _ = O.describe(s[:31])
                 want = o.get(int(key[0])) if valid[0] else (False, 0)
                 assert t.get(s[:31]) == want, ("get", s[:31])
+
+
+def _offsets(seqs):
+    off = np.zeros(len(seqs) + 1, np.uint64)
+    np.cumsum([len(s) + 1 for s in seqs], out=off[1:])
+    return off
+
+
+def run_device_program(api, case):
+    """A seeded random program through the device library's own entry points
+    (include/kache_amd.h): device-text inserts, oriented inserts, pipelined
+    batches, updates and counts (totals, per batch, per sequence), and host
+    texts; each against the oracle as the calls of the C ABI it stands for (a
+    batch is one call of the growth schedule)."""
+    import ctypes as C
+    rng = np.random.default_rng(0xD5EED + case)
+    width = int(rng.choice([0, 1, 2, 4, 8]))
+    vmax = (1 << (8 * width)) - 1 if width else 0
+    expected = int(rng.choice([10, 300, 5_000, 40_000]))
+    pow2 = bool(rng.random() < 0.3)
+    genome = G.genome(int(rng.integers(3_000, 40_000)), 9000 + case)
+    groups = G.minimizer_groups(3, 120, 9500 + case) if rng.random() < 0.5 else []
+    calls = int(rng.integers(6, 14))
+    print(f"device case {case}: width {width}, expected {expected}, pow2 {pow2}, genome {len(genome)}, "
+          f"groups {bool(groups)}, calls {calls}", flush=True)
+    L = api.lib
+    with capi.Table(api, width, expected, 0.8, pow2, 2) as t:
+        o = O.OTable(width)
+        sched = O.Sched(expected, 0.8, pow2)
+        u = t.register()
+        for step in range(calls):
+            seqs = sequences(rng, genome, groups, int(rng.integers(1, 60)))
+            buf, nbytes = capi.device_text(api, seqs)
+            off = _offsets(seqs)
+            kind = rng.random()
+            if kind < 0.2:
+                op = int(rng.integers(0, 6)) if width else KEEP
+                x = operand(rng, op, vmax) if width else 0
+                want = o.insert(seqs, op, x)
+                sched.call(u, want)
+                assert t.insert_device(buf.ptr, nbytes, op, x, u) == want, (step, "insert_device")
+            elif kind < 0.35:
+                # Pipelined batches: bounds at random sequence starts.
+                cuts = np.unique(rng.integers(0, len(seqs) + 1, size=int(rng.integers(0, 6))))
+                idx = [0] + [int(c) for c in cuts if 0 < c < len(seqs)] + [len(seqs)]
+                bounds = off[idx]
+                op = int(rng.integers(0, 6)) if width else KEEP
+                x = operand(rng, op, vmax) if width else 0
+                got = t.insert_batches(buf.ptr, bounds, op, x, u)
+                for i in range(len(idx) - 1):
+                    want = o.insert(seqs[idx[i]:idx[i + 1]], op, x)
+                    sched.call(u, want)
+                    assert int(got[i]) == want, (step, "insert_batches", i)
+            elif kind < 0.45 and width:
+                op = int(rng.choice([ADD, OR, MAX]))
+                xf, xr = operand(rng, op, vmax), operand(rng, op, vmax)
+                want = o.insert_oriented(seqs, op, xf, xr)
+                sched.call(u, want)
+                assert t.insert_oriented_device(buf.ptr, nbytes, op, xf, xr, u) == want, (step, "oriented_device")
+            elif kind < 0.55:
+                op = int(rng.integers(0, 6)) if width else KEEP
+                x = operand(rng, op, vmax) if width else 0
+                text = b"".join(s + b"\n" for s in seqs)
+                want = o.insert(seqs, op, x)
+                sched.call(u, want)
+                assert t.insert_text(text, op, x, u) == want, (step, "insert_text")
+            elif kind < 0.65 and width:
+                op = int(rng.integers(0, 6))
+                x = operand(rng, op, vmax)
+                want = o.update(seqs, op, x)
+                if rng.random() < 0.5:
+                    assert t.update_device(buf.ptr, nbytes, op, x) == want, (step, "update_device")
+                else:
+                    assert t.update_text(b"".join(s + b"\n" for s in seqs), op, x) == want, (step, "update_text")
+            else:
+                counts = o.counts(seqs)[0]
+                assert t.count_device(buf.ptr, nbytes) == int(counts.sum()), (step, "count_device")
+                per = np.zeros(max(len(seqs), 1), np.uint64)
+                api.check(L.kache_amd_count_seqs_device(t.h, buf.ptr, int(off[-1]), off.ctypes.data, len(seqs), per.ctypes.data),
+                          "kache_amd_count_seqs_device")
+                assert np.array_equal(per[:len(seqs)], counts), (step, "count_seqs_device")
+                h, p = t.count_batches(buf.ptr, np.array([0, off[-1]], np.uint64))
+                assert int(h[0]) == int(counts.sum()), (step, "count_batches")
+            buf.free()
+            assert t.stats()["capacity"] == sched.cap * 32, (step, "capacity")
+            assert t.size() == o.size(), (step, "size")
+        same_contents(t, o)

@@ -1,6 +1,7 @@
 """The seeded random programs of tests/programs.py through the device
-library, against the oracle (the same programs run against the reference
-library in test_oracle.py)."""
+library, against the oracle: C-ABI programs (the same ones run against the
+reference library in test_oracle.py) and programs of the library's own
+device-text, batch and host-text entry points."""
 import pytest
 
 import kache_hash_amd as KA
@@ -17,3 +18,8 @@ def api():
 @pytest.mark.parametrize("case", range(programs.CASES))
 def test_random_program(api, case):
     programs.run_program(api, case)
+
+
+@pytest.mark.parametrize("case", range(programs.DEVICE_CASES))
+def test_random_device_program(api, case):
+    programs.run_device_program(api, case)
