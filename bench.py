@@ -378,18 +378,22 @@ def served_probe(a, KA, capi, api, t, u):
     serving workload's mix and shapes drawn from the same genome (count,
     lookup, update +1, upsert +1), run by the serving loop and by plain
     per-request calls, closed loop and at Poisson arrivals of a quarter, half
-    and three quarters of the plain path's capacity (two alternating runs of
-    each per rate, latencies pooled).  After the timed region; its upserts
+    and three quarters of the plain path's capacity (four alternating runs of
+    each per rate, latencies pooled: a run's p99 is its sixth-slowest of 600
+    requests and moves by a few percent from run to run).  After the timed region; its upserts
     change the table."""
     g = capi.DeviceBuffer(api, a.genome)
     api.check(api.lib.kache_amd_synth_genome(g.ptr, a.genome, a.seed), "synth_genome")
     genome = g.download(a.genome)
     g.free()
     reqs = serve_requests(capi, genome, a.served_requests, a.seed + 5, a.genome)
-    made = [capi.make_request(8, k, b, capi.ADD, 1) for k, b in reqs]
+    # Output arrays in page-locked memory (kache_amd_host_alloc), as a server's
+    # response buffers would be: lookups are copied into them by DMA directly.
+    made = [capi.make_request(8, k, b, capi.ADD, 1, api=api) for k, b in reqs]
     for m in made:
-        for arr in m[1].values():
-            arr.fill(1)
+        for key, arr in m[1].items():
+            if not key.startswith("_"):
+                arr.fill(1)
     WAVE = 64 << 20
     serve_run(capi, t, u, reqs, made, np.zeros(len(reqs)), WAVE, 1 << 20)     # warm-up
     ro = [m for m, (k, _) in zip(made, reqs) if k in (capi.REQ_COUNT, capi.REQ_LOOKUP)]
@@ -404,7 +408,7 @@ def served_probe(a, KA, capi, api, t, u):
             serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro)       # untimed open-loop warm-up of each
             serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro)
         runs_s, runs_p = [], []
-        for _ in range(2):      # alternating, on the same arrivals; latencies pooled
+        for _ in range(4):      # alternating, on the same arrivals; latencies pooled
             runs_s.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1 << 20, ro))
             runs_p.append(serve_run(capi, t, u, reqs, made, arr, WAVE, 1, ro))
         at_s, at_p = pooled(runs_s), pooled(runs_p)
@@ -621,12 +625,15 @@ def serve_main(a, KA, capi, api):
     pos = sum(b.positions() for _, b in reqs)
     print(f"serve: {len(reqs)} requests, {pos} k-mer positions", file=sys.stderr, flush=True)
     WAVE = 64 << 20
-    # Requests (and their output arrays) are made once and reused by every run.
-    made = [capi.make_request(4, k, b, capi.ADD, 1) for k, b in reqs]
-    wmade = [capi.make_request(4, k, b, capi.ADD, 1) for k, b in warm]
+    # Requests (and their output arrays) are made once and reused by every run;
+    # the outputs are page-locked (kache_amd_host_alloc), as a server's response
+    # buffers would be, so lookups are copied into them by DMA directly.
+    made = [capi.make_request(4, k, b, capi.ADD, 1, api=api) for k, b in reqs]
+    wmade = [capi.make_request(4, k, b, capi.ADD, 1, api=api) for k, b in warm]
     for m in made + wmade:
-        for arr in m[1].values():
-            arr.fill(1)         # fault the output pages in before any timing
+        for key, arr in m[1].items():
+            if not key.startswith("_"):
+                arr.fill(1)
     serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1 << 20)       # warm-up: the first upserts double the table
     serve_run(capi, t, u, warm, wmade, np.zeros(len(warm)), WAVE, 1)
     ro = [m for m, (k, _) in zip(wmade, warm) if k in (capi.REQ_COUNT, capi.REQ_LOOKUP)]

@@ -235,7 +235,10 @@ int kache_amd_copy(void* dst, const void* src, uint64_t bytes, int to_device) KA
 int kache_amd_device_fill(void* ptr, uint8_t value, uint64_t bytes) KACHE_NOEXCEPT;
 /* Waits for all work on the current device (hipDeviceSynchronize). */
 int kache_amd_device_sync(void) KACHE_NOEXCEPT;
-/* Pinned host memory (hipHostMalloc): copies from and to it run at full link speed. */
+/* Pinned host memory (hipHostMalloc): copies from and to it run at full link
+ * speed.  Lookup outputs (kache_lookup, kache_amd_lookup_groups, served
+ * lookups) placed in it are written by DMA directly, without the bounce
+ * through the library's own pinned chunks that pageable outputs take. */
 int kache_amd_host_alloc(uint64_t bytes, void** ptr) KACHE_NOEXCEPT;
 int kache_amd_host_free(void* ptr) KACHE_NOEXCEPT;
 

@@ -511,28 +511,61 @@ class DeviceBuffer:
         return out.view(dtype)
 
 
-def make_request(width, kind, seqs, op=KEEP, x=0, values=True, fwd=True):
+class HostArray:
+    """A zeroed numpy array in page-locked host memory from the library
+    (kache_amd_host_alloc), which the device's copy engines write directly;
+    the memory is freed with this object (keep it while the array is used)."""
+
+    def __init__(self, api, n, dtype):
+        self.api = api
+        nbytes = max(n, 1) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        api.check(api.lib.kache_amd_host_alloc(nbytes, C.byref(p)), "kache_amd_host_alloc")
+        self.ptr = p.value
+        self.array = np.frombuffer((C.c_uint8 * nbytes).from_address(self.ptr), np.uint8).view(dtype)[:max(n, 1)]
+        self.array.fill(0)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.api.lib.kache_amd_host_free(self.ptr)
+            self.ptr = None
+
+
+def make_request(width, kind, seqs, op=KEEP, x=0, values=True, fwd=True, api=None):
     """A serving request over a table of `width`-byte values and the arrays it
-    writes: (Request, outputs, batch); reusable across servers and runs."""
+    writes: (Request, outputs, batch); reusable across servers and runs.  With
+    `api`, the output arrays are page-locked (HostArray, kept in
+    outputs["_host"]), so lookups are copied into them by DMA directly."""
     b = as_batch(seqs)
     r = Request(kind=kind, op=op, x=x, seqs=b.c_seqs, lens=b.c_lens, n=b.n)
     out = {}
+    held = []
+
+    def arr(n, dtype):
+        if api is None:
+            return np.zeros(n, dtype)
+        h = HostArray(api, n, dtype)
+        held.append(h)
+        return h.array
+
     if kind == REQ_COUNT:
-        out["counts"] = np.zeros(max(b.n, 1), np.uint64)
+        out["counts"] = arr(max(b.n, 1), np.uint64)
         r.counts = out["counts"].ctypes.data
     elif kind == REQ_LOOKUP:
         p = max(b.positions(), 1)
-        out["found"] = np.zeros(p, np.uint8)
+        out["found"] = arr(p, np.uint8)
         r.found = out["found"].ctypes.data
         if values and width:
-            out["values"] = np.zeros(p * width, np.uint8)
+            out["values"] = arr(p * width, np.uint8)
             r.values = out["values"].ctypes.data
         if fwd:
-            out["fwd"] = np.zeros(p, np.uint8)
+            out["fwd"] = arr(p, np.uint8)
             r.fwd = out["fwd"].ctypes.data
     else:
-        out["counts"] = np.zeros(1, np.uint64)
+        out["counts"] = arr(1, np.uint64)
         r.counts = out["counts"].ctypes.data
+    if held:
+        out["_host"] = held
     return r, out, b
 
 

@@ -384,6 +384,19 @@ public:
     }
 };
 
+// Whether p lies in page-locked host memory (hipHostMalloc, as
+// kache_amd_host_alloc gives), which the copy engines reach directly.
+static bool pinned_host(const void* p)
+{
+    hipPointerAttribute_t a{};
+    if(hipPointerGetAttributes(&a, p) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 // Copies n bytes, in parallel when large.
 void par_copy(char* dst, const char* src, uint64_t n)
 {
@@ -2061,15 +2074,16 @@ struct kache_table
     // With `dev_text`, the sequences are already there, at offsets `dev_off`
     // (as per_sequence_device takes them), and seqs / lens are unused.
     // Copies `n` bytes of device memory to the caller's memory, in stream
-    // order: a small copy is only enqueued (the caller synchronizes the
-    // stream before it reads); a large one goes through two pinned chunks,
-    // the copy engine filling chunk i + 1 while the packing threads copy
-    // chunk i out, several times the rate of a copy into pageable memory,
-    // and has ended when this returns.
+    // order: a small copy, or one into page-locked memory (kache_amd_host_alloc),
+    // which the copy engine writes directly, is only enqueued (the caller
+    // synchronizes the stream before it reads); a large one into pageable
+    // memory goes through two pinned chunks, the copy engine filling chunk
+    // i + 1 while the packing threads copy chunk i out, several times the
+    // rate of a copy into pageable memory, and has ended when this returns.
     void download(void* dst, const void* src, uint64_t n) const
     {
         constexpr uint64_t CHUNK = uint64_t(8) << 20;
-        if(n < 2 * CHUNK)
+        if(n < 2 * CHUNK || pinned_host(dst))
         {
             HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream));
             return;

@@ -351,3 +351,37 @@ def test_serve_hbm_budget_refuses_an_oversized_request(api):
     p = len(f)
     assert np.array_equal(out["found"][:p], f) and np.array_equal(out["fwd"][:p], fw)
     t.close()
+
+
+def test_lookups_into_page_locked_outputs(api):
+    """Lookup outputs in page-locked memory (kache_amd_host_alloc) are copied
+    by DMA directly, others through pinned chunks: both give the outputs of
+    kache_lookup, for requests above and below the chunked path's 16 MiB,
+    served and through kache_amd_lookup_groups."""
+    import ctypes as C
+    rng = np.random.default_rng(31)
+    g = G.genome(3_000_000, 32)
+    t = capi.Table(api, 4, expected_keys=4_000_000)
+    u = t.register()
+    t.insert([g[i:i + 100_000] for i in range(0, len(g), 100_000)], capi.ADD, 1, u)
+    big = [g[int(s):int(s) + 100_000] for s in rng.integers(0, len(g) - 100_000, 60)]      # ~6M positions: 24 MB of values
+    small = random_seqs(rng, g, 50)
+    for seqs in (big, small):
+        f, v, fw = t.lookup(seqs)
+        p = len(f)
+        for pinned in (False, True):
+            r, o, b = capi.make_request(4, capi.REQ_LOOKUP, seqs, api=api if pinned else None)
+            P = C.c_void_p * 1
+            first = np.zeros(1, np.uint64)
+            api.check(api.lib.kache_amd_lookup_groups(t.h, b.c_seqs, b.c_lens, b.n, first.ctypes.data, 1,
+                                                      P(o["found"].ctypes.data), P(o["values"].ctypes.data),
+                                                      P(o["fwd"].ctypes.data)), "lookup_groups")
+            assert np.array_equal(o["found"][:p], f) and np.array_equal(o["fwd"][:p], fw)
+            assert np.array_equal(o["values"][:4 * p].view(np.uint32).astype(np.uint64), v)
+            r, o, b = capi.make_request(4, capi.REQ_LOOKUP, seqs, api=api if pinned else None)
+            with capi.Server(t, u, 64 << 20, 1 << 20) as s:
+                res = s.replay([(r, o, b)], np.zeros(1))
+            assert res[0]["status"] == 0
+            assert np.array_equal(o["found"][:p], f) and np.array_equal(o["fwd"][:p], fw)
+            assert np.array_equal(o["values"][:4 * p].view(np.uint32).astype(np.uint64), v)
+    t.close()
