@@ -597,9 +597,10 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def request(self, kind, seqs, op=KEEP, x=0, values=True, fwd=True):
-        """A request and the arrays it writes: (Request, outputs, batch)."""
-        return make_request(self.table.width, kind, seqs, op, x, values, fwd)
+    def request(self, kind, seqs, op=KEEP, x=0, values=True, fwd=True, pinned=False):
+        """A request and the arrays it writes: (Request, outputs, batch); with
+        `pinned`, the outputs are page-locked (see make_request)."""
+        return make_request(self.table.width, kind, seqs, op, x, values, fwd, api=self.api if pinned else None)
 
     def submit(self, req):
         """Submits (Request, outputs, batch) from request(); returns the ticket."""

@@ -112,10 +112,12 @@ def same(a, b):
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
 
 
-def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, prefill=None, gaps=None):
+def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, prefill=None, gaps=None, pinned=False):
     """Serves `reqs` (one submitter: request i after a pause of gaps[i]
     seconds, if given), then checks every output against the oracle and a
-    serial device table run in served order.  Returns the served results."""
+    serial device table run in served order.  With `pinned`, outputs are
+    page-locked (lookups copied into them by DMA directly).  Returns the
+    served results."""
     t = capi.Table(api, width, expected_keys=expected)
     u = t.register()
     o = O.OTable(width)
@@ -126,7 +128,7 @@ def serve(api, width, reqs, wave_bytes, max_wave, threads=1, expected=20_000, pr
         ser.insert(prefill, capi.ADD if width else capi.KEEP, 1 if width else 0, su)
         o.insert(prefill, capi.ADD if width else 0, 1 if width else 0)
     with capi.Server(t, u, wave_bytes, max_wave) as s:
-        made = [s.request(k, seqs, op, x) for k, seqs, op, x in reqs]
+        made = [s.request(k, seqs, op, x, pinned=pinned) for k, seqs, op, x in reqs]
         tickets = [None] * len(reqs)
         if threads == 1:
             for i, r in enumerate(made):
@@ -173,6 +175,16 @@ def test_serve_mixed_wide_waves(api):
     print("serve u32, 300 requests, one wave budget of 64 MiB", flush=True)
     res, stats = serve(api, 4, reqs, 64 << 20, 1 << 20, prefill=[g[:50_000]])
     assert stats["waves"] < len(reqs)          # requests merged
+
+
+def test_serve_page_locked_outputs(api):
+    """Served lookups into page-locked outputs (copied by DMA directly), in
+    waves mixing every kind, with the table doubling meanwhile."""
+    g = G.genome(300_000, 57)
+    reqs = make_requests(8, g, 400, 8, p=(0.2, 0.45, 0.1, 0.25), max_seqs=60)
+    print("serve u64, 400 requests, page-locked outputs", flush=True)
+    res, stats = serve(api, 8, reqs, 64 << 20, 1 << 20, expected=2_000, prefill=[g[:40_000]], pinned=True)
+    assert stats["waves"] < len(reqs)
 
 
 def test_serve_small_waves_and_oversized_requests(api):
@@ -385,3 +397,4 @@ def test_lookups_into_page_locked_outputs(api):
             assert np.array_equal(o["found"][:p], f) and np.array_equal(o["fwd"][:p], fw)
             assert np.array_equal(o["values"][:4 * p].view(np.uint32).astype(np.uint64), v)
     t.close()
+
