@@ -40,6 +40,7 @@ LINES_PER_KMER = 1 / 5.29 + 16 / 64
 
 # Timed passes whose launches are bracketed by HIP events: one in PROFILE_EVERY.
 PROFILE_EVERY = 4
+GROWTH = {"auto": 0, "in-place": 1, "out-of-place": 2}    # KACHE_AMD_GROW_*
 
 
 def transaction_roof(kmers_per_s):
@@ -91,6 +92,8 @@ def parse():
     p.add_argument("--served-requests", type=int, default=600,
                    help="count-map: requests of the served-latency probe on the built table (0: none)")
     p.add_argument("--blocks-per-cu", type=int, default=0)
+    p.add_argument("--growth", choices=["auto", "in-place", "out-of-place"], default="auto",
+                   help="how tables double (kache_amd_set_growth)")
     p.add_argument("--seed", type=int, default=None)
     p.add_argument("--workload", choices=["count-map", "serve", "counter", "batched"], default="count-map")
     p.add_argument("--batched-gbp", type=float, default=8.0)
@@ -256,6 +259,7 @@ def main():
         """One step: a fresh table sized for the expected keys, every batch
         upserted (+1, new keys at 1) as one pipelined call; returns the table."""
         t = KA.table(width=8, expected_keys=a.expected)
+        t.set_growth(GROWTH[a.growth])
         if a.blocks_per_cu:
             t.set_blocks_per_cu(a.blocks_per_cu)
         t.profile(profile)
@@ -349,11 +353,13 @@ def main():
                                "included), lookups of 2M fresh reads",
                    "step": "one full pass: table creation, every batch, the doublings, table release",
                    "reads": n_reads, "batches": n_batches, "batch_reads": a.batch, "expected_keys": a.expected,
-                   "genome_bp": a.genome},
+                   "genome_bp": a.genome, "growth": a.growth},
         "lookups_per_s": look_kmers / dt_look,
         "lookup_hits": hits, "lookup_kmers": look_kmers,
         "timed_kmers": timed_kmers, "kmers_per_step": kmers_per_pass,
         "keys": stats["keys"], "capacity": stats["capacity"], "resizes": info["resizes"],
+        "in_place_doublings": info["in_place_doublings"], "store_mapped_bytes": info["store_mapped_bytes"],
+        "last_doubling_away_keys": info["last_away_keys"],
         "placement_pct": [stats["primary_pct"], stats["secondary_pct"], stats["overflow_pct"]],
         "kernel_ms": {"insert": ins_ms, "split": split_ms, "reinsert": re_ms, "query": q_ms,
                       "insert_launches": ins_launches, "query_launches": q_launches, "profiled_steps": profiled},

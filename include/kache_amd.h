@@ -165,6 +165,30 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
  * tries the doubling again. */
 int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
 
+/* How the table doubles (the reference doubles its store in place,
+ * include/Streaming_Kmer_Hash_Table.hpp:1473-1681 over include/utility.hpp:
+ * 87-116; this entry point has no counterpart there).  In place: the store is
+ * an address range reserving room to grow (HIP virtual memory); a doubling
+ * maps its second half and splits the buckets from the top, so it needs the
+ * new half, the new overflow table, a copy of 1/64 of the store and a list of
+ * the keys away from their primary bucket (16 bytes each) beside the table,
+ * instead of a second, doubled store.  Out of place: a new store of twice the
+ * size, the old one freed after.  Results are the same either way.
+ *   KACHE_AMD_GROW_AUTO          in place for stores of 2 MiB and more (the default)
+ *   KACHE_AMD_GROW_IN_PLACE      in place at every size; a store that cannot
+ *                                grow in place moves out of place once, into one that can
+ *   KACHE_AMD_GROW_OUT_OF_PLACE  always out of place
+ * away_entries: the first size of an in-place doubling's list of away keys
+ * (0: the default, an eighth of the keys and at least 65,536; at least 256
+ * otherwise); a list that fills up is grown and the doubling goes on. */
+enum
+{
+    KACHE_AMD_GROW_AUTO = 0,
+    KACHE_AMD_GROW_IN_PLACE = 1,
+    KACHE_AMD_GROW_OUT_OF_PLACE = 2
+};
+int kache_amd_set_growth(kache_table* t, uint32_t mode, uint64_t away_entries) KACHE_NOEXCEPT;
+
 /* Bucket stores, overflow tables and doubling arrays that tables free are kept
  * for reuse by later tables and growth on the same device (the driver wipes
  * memory it gets back before handing it out again, which for tables of tens
@@ -225,6 +249,13 @@ typedef struct kache_amd_info
     uint64_t next_doubling_bytes;   /* HBM the next doubling needs beside the table's arrays */
     uint64_t device_free_bytes;     /* free HBM of the table's device, arrays kept for reuse included */
     uint64_t hbm_budget;            /* kache_amd_set_hbm_budget (0: none) */
+    uint64_t store_mapped_bytes;    /* HBM mapped for the bucket store (a store taken from kept memory may hold more than its buckets) */
+    uint64_t store_reserved_bytes;  /* address space reserved for it (0: a store that cannot grow in place) */
+    uint64_t in_place_doublings;    /* doublings done in place */
+    uint64_t away_list_regrows;     /* times an in-place doubling's list of away keys grew */
+    uint64_t last_away_keys;        /* keys the last in-place doubling listed (away from their primary bucket) */
+    uint32_t growth_mode;           /* kache_amd_set_growth */
+    uint32_t reserved0;
 } kache_amd_info;
 int kache_amd_get_info(const kache_table* t, kache_amd_info* out) KACHE_NOEXCEPT;
 

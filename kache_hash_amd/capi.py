@@ -36,7 +36,12 @@ class Info(C.Structure):
     _fields_ = [("buckets", C.c_uint64), ("bucket_bytes", C.c_uint64), ("overflow_slots", C.c_uint64),
                 ("overflow_keys", C.c_uint64), ("resizes", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("value_bytes", C.c_uint64), ("next_doubling_bytes", C.c_uint64), ("device_free_bytes", C.c_uint64),
-                ("hbm_budget", C.c_uint64)]
+                ("hbm_budget", C.c_uint64), ("store_mapped_bytes", C.c_uint64), ("store_reserved_bytes", C.c_uint64),
+                ("in_place_doublings", C.c_uint64), ("away_list_regrows", C.c_uint64), ("last_away_keys", C.c_uint64),
+                ("growth_mode", C.c_uint32),
+                ("reserved0", C.c_uint32)]
+
+GROW_AUTO, GROW_IN_PLACE, GROW_OUT_OF_PLACE = range(3)   # KACHE_AMD_GROW_*
 
 
 REQ_COUNT, REQ_LOOKUP, REQ_UPDATE, REQ_INSERT = range(4)   # KACHE_AMD_REQ_*
@@ -138,6 +143,7 @@ AMD_SYMBOLS = {
     "kache_amd_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kache_amd_set_blocks_per_cu": (C.c_int, [_P, C.c_uint32]),
     "kache_amd_set_hbm_budget": (C.c_int, [_P, C.c_uint64]),
+    "kache_amd_set_growth": (C.c_int, [_P, C.c_uint32, C.c_uint64]),
     "kache_amd_trim_cache": (C.c_int, [_P]),
     "kache_amd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
     "kache_amd_last_error": (C.c_char_p, []),
@@ -475,6 +481,12 @@ class Table:
     def set_hbm_budget(self, nbytes):
         """Most bytes of HBM the table may hold (0: no limit but the device's)."""
         self.api.check(self.api.lib.kache_amd_set_hbm_budget(self.h, nbytes), "kache_amd_set_hbm_budget")
+
+    def set_growth(self, mode, away_entries=0):
+        """How the table doubles: GROW_AUTO, GROW_IN_PLACE or GROW_OUT_OF_PLACE;
+        away_entries sets the first size of an in-place doubling's list of
+        away keys (0: the default)."""
+        self.api.check(self.api.lib.kache_amd_set_growth(self.h, mode, away_entries), "kache_amd_set_growth")
 
     def set_blocks_per_cu(self, blocks):
         self.api.check(self.api.lib.kache_amd_set_blocks_per_cu(self.h, blocks), "kache_amd_set_blocks_per_cu")

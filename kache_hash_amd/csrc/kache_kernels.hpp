@@ -475,6 +475,244 @@ __global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableVi
     block_add(failed, &cnt->failed, red);
 }
 
+// ---------------------------------------------------------------------------
+// Resize in place (the reference also grows its store in place and splits
+// b -> 2b, 2b + 1 from the top, include/Streaming_Kmer_Hash_Table.hpp:
+// 1473-1681, include/utility.hpp:87-116).  The store's address range is
+// mapped to twice its size first; old bucket b's primaries then move to new
+// buckets 2b and 2b + 1 of the same range.  Phases run from the top: phase
+// [lo, hi) reads old buckets lo .. hi - 1 and writes new buckets 2 lo ..
+// 2 hi - 1, all at or above hi (2 lo >= hi): in the new half, or old buckets
+// that an earlier phase has already read.  A phase never writes its own
+// sources, nor those of a later phase.  The lowest K buckets are copied aside
+// before the first phase and split last, from the copy, so that there are
+// log2(cap / K) + 1 phases rather than log2(cap) + 1.
+//
+// The keys away from their primary bucket cannot wait in their old slots
+// (those are overwritten), so the split lists them with their values in the
+// away list: `regions` regions of `region` entries, region r shared by the
+// waves whose index is r modulo `regions`.  A wave's step takes entries for
+// its away keys from its region's cursor by one returning atomic, and writes
+// them during the next step, so that the atomic's round trip overlaps a
+// step's work instead of stalling it (waiting for it on every step made the
+// split 60% slower).  A phase that finds its region full sets `fail`
+// (1 + the phase), and the phases after it return at once; the host then
+// grows the list and runs that phase again from the cursors it had before it
+// (kept by k_away_snap).
+
+constexpr uint32_t AWAY_REGIONS = 256;
+constexpr uint32_t AWAY_CURSOR_STRIDE = 8;     // words between cursors: one 64-byte line each
+
+struct AwayList
+{
+    uint64_t* keys;       // stored keys (the complement of the key)
+    uint64_t* vals;       // their value words (maps)
+    uint64_t region;      // entries per region
+    uint64_t* cursor;     // regions cursors, AWAY_CURSOR_STRIDE words apart
+    uint32_t* fail;       // 0, or 1 + the first phase that found its region full
+    uint32_t regions;
+};
+
+template <int VB>
+__global__ __launch_bounds__(TILE) void k_split_in_place(const uint8_t* src, TableView N, uint64_t old_cap, uint64_t lo, uint64_t hi, AwayList L, uint32_t phase)
+{
+    if(__hip_atomic_load(L.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return;
+    constexpr uint32_t BB = Geo<VB>::bucket_bytes, IMG = 2 * BB;
+    __shared__ __align__(16) uint8_t images[TILE / 64][2][IMG];
+    const uint32_t lane = threadIdx.x & 63, j = lane & 31;
+    const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    const uint64_t pairs = (hi - lo + 1) / 2;
+    const uint64_t stride = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6;
+    const uint32_t r = static_cast<uint32_t>(w % L.regions);
+    uint64_t* cursor = L.cursor + uint64_t(r) * AWAY_CURSOR_STRIDE;
+    uint64_t* rkeys = L.keys + r * L.region;
+    uint64_t* rvals = VB != 0 ? L.vals + r * L.region : nullptr;
+    // The last step's away keys, waiting for their entries: p_at (lane 0)
+    // is the cursor before them, p_n (wave-uniform) their number.
+    uint64_t p_sk = 0, p_val = 0, p_at = 0;
+    uint32_t p_n = 0, p_rank = 0;
+    bool p_aw = false;
+    // Writes them; false once the region is full (the wave then stops: the
+    // phase runs again).
+    const auto flush = [&]
+    {
+        if(p_n == 0)
+            return true;
+        const uint64_t at = readlane64(p_at, 0);
+        if(at + p_n > L.region)
+        {
+            if(lane == 0)
+            {
+                uint32_t none = 0;
+                __hip_atomic_compare_exchange_strong(L.fail, &none, phase + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return false;
+        }
+        if(p_aw)
+        {
+            const uint64_t i = at + p_rank;
+            rkeys[i] = p_sk;
+            if constexpr(VB != 0)
+                rvals[i] = p_val;
+        }
+        return true;
+    };
+    // Slot j of old bucket lo + 2w + lane / 32, read a pair ahead of its use.
+    struct Old
+    {
+        uint64_t sk, val;
+        uint32_t c;
+    };
+    const auto read = [&](uint64_t w)
+    {
+        Old o{0, 0, 0};
+        const uint64_t b = lo + 2 * w + (lane >> 5);
+        if(w < pairs && b < hi)
+        {
+            uint8_t* ob = const_cast<uint8_t*>(src) + b * BB;
+            const Slot<VB> os = slot_ref<VB>(ob, j);
+            o.c = ob[j], o.sk = os.k[0];
+            if constexpr(VB != 0)
+                o.val = *os.v();
+        }
+        return o;
+    };
+    Old next = read(w);
+    for(; w < pairs; w += stride)
+    {
+        const Old cur = next;
+        next = read(w + stride);
+        const uint64_t b = lo + 2 * w + (lane >> 5);
+        bool lo_half = false, hi_half = false, aw = false;
+        const uint64_t sk = cur.sk, val = cur.val;
+        uint8_t c = static_cast<uint8_t>(cur.c);
+        if(c && !is_key(sk))
+            c = 0;      // a tombstone: dropped
+        if(c)
+        {
+            const uint64_t h = minimizer_hash(~sk);
+            if(bucket_of(h, old_cap) == b)
+            {
+                hi_half = (bucket_of(h, N.cap) == 2 * b + 1);
+                lo_half = !hi_half;
+            }
+            else
+                aw = true;
+        }
+        const uint64_t m_lo = __ballot(lo_half) & half_mask, m_hi = __ballot(hi_half) & half_mask, m_away = __ballot(aw);
+        uint8_t* img = images[threadIdx.x >> 6][lane >> 5];
+        wave_sync();    // the previous pair's image has been read out
+        for(uint32_t k = j; k < IMG / 16; k += 32)
+            reinterpret_cast<uint4*>(img)[k] = make_uint4(0, 0, 0, 0);
+        wave_sync();
+        if(lo_half || hi_half)
+        {
+            const uint32_t rank = __popcll((lo_half ? m_lo : m_hi) & below);
+            uint8_t* nb = img + (lo_half ? 0 : BB);
+            const Slot<VB> ns = slot_ref<VB>(nb, rank);
+            ns.k[0] = sk;
+            if constexpr(VB != 0)
+                *ns.v() = static_cast<typename Geo<VB>::V>(val);
+            nb[rank] = c;
+        }
+        wave_sync();
+        if(b < hi)
+        {
+            uint4* dst = reinterpret_cast<uint4*>(bucket_at<VB>(N, 2 * b));
+            for(uint32_t k = j; k < IMG / 16; k += 32)
+                dst[k] = reinterpret_cast<const uint4*>(img)[k];
+        }
+        // The previous step's away keys, now that their entries are known;
+        // then this step's entries are taken.
+        if(!flush())
+            return;
+        p_n = __popcll(m_away);
+        if(p_n)
+        {
+            if(lane == 0)
+                p_at = __hip_atomic_fetch_add(cursor, uint64_t(p_n), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p_aw = aw, p_rank = __popcll(m_away & below), p_sk = sk, p_val = val;
+        }
+    }
+    flush();
+}
+
+// The away list's cursors before a phase: `regions` words to `snap`.
+__global__ void k_away_snap(const uint64_t* cursor, uint32_t regions, uint64_t* snap)
+{
+    for(uint32_t r = threadIdx.x; r < regions; r += blockDim.x)
+        snap[r] = cursor[uint64_t(r) * AWAY_CURSOR_STRIDE];
+}
+
+// The keys of the away list and of the old overflow table into the doubled
+// table, with the insertion policy and the reinsert's claims (insert_new_wave).
+// Each key inserted is cleared from its list entry or old overflow slot, so
+// that a run which finds the new overflow table full can be repeated, after
+// that table has grown, for the keys left.
+template <int VB>
+__global__ __launch_bounds__(TILE) void k_reinsert_away(AwayList L, uint64_t* old_ov, uint64_t old_ov_slots, TableView N, Counters* cnt)
+{
+    __shared__ uint64_t red[TILE / 64];
+    uint64_t ov_n = 0, failed = 0;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t per_region = (L.region + 63) / 64;
+    const uint64_t list_chunks = uint64_t(L.regions) * per_region;
+    const uint64_t chunks = list_chunks + (old_ov_slots + 63) / 64;
+    const uint64_t waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    for(uint64_t ch = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; ch < chunks; ch += waves)
+    {
+        uint64_t* at = nullptr;
+        uint64_t sk = 0, val = 0;
+        if(ch < list_chunks)
+        {
+            const uint64_t r = ch / per_region, i = (ch % per_region) * 64 + lane;
+            const uint64_t used = L.cursor[r * AWAY_CURSOR_STRIDE];
+            if(i < used && i < L.region)
+            {
+                at = L.keys + r * L.region + i;
+                sk = *at;
+                if constexpr(VB != 0)
+                    val = L.vals[r * L.region + i];
+            }
+        }
+        else
+        {
+            const uint64_t i = (ch - list_chunks) * 64 + lane;
+            if(i < old_ov_slots)
+            {
+                const Slot<VB> os = ov_ref<VB>(old_ov, i);
+                at = os.k;
+                sk = *at;
+                if constexpr(VB != 0)
+                    if(sk)
+                        val = *os.v();
+            }
+        }
+        const bool active = sk != 0;
+        Slot<VB> slot;
+        bool in_ov;
+        const uint64_t key = ~sk;
+        const Res res = insert_new_wave<VB>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
+        if(active)
+        {
+            if(res == RES_FULL)
+                failed++;
+            else
+            {
+                ov_n += in_ov;
+                if constexpr(VB != 0)
+                    *slot.v() = static_cast<typename Geo<VB>::V>(val);
+                *at = 0;
+            }
+        }
+    }
+    block_add(ov_n, &cnt->ov_inserted, red);
+    block_add(failed, &cnt->failed, red);
+}
+
 // Overflow growth: the old overflow table's keys into a new one
 // (include/Streaming_Kmer_Hash_Table.hpp:1412-1443).
 template <int VB>

@@ -41,7 +41,11 @@ def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
     ot = O.OTable(width)
     u = t.register()
     cap0 = t.stats()["capacity"]
-    t.set_hbm_budget(t.info()["device_bytes"] + (32 << 20))    # room for overflow growth, not for a doubling (>= 50 MB)
+    # Room for overflow growth (<= 16 MB), not for a doubling: half what the
+    # next doubling needs beside the table (in place: the store's second half,
+    # >= 25 MB, and the new overflow table).
+    info = t.info()
+    t.set_hbm_budget(info["device_bytes"] + info["next_doubling_bytes"] // 2)
     refused = None
     for i in range(0, len(reads), 4000):
         part = reads[i:i + 4000]

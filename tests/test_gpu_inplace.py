@@ -1,0 +1,181 @@
+"""Doubling in place (kache_amd_set_growth): the store's address range is
+mapped to twice its size and split from the top, the keys away from their
+primary bucket go through the away list.  Every dump, value, lookup, per-call
+return and capacity must equal the oracle's (and its growth schedule's), and
+equal what the same calls give with out-of-place doublings; an away list too
+small for the keys it must hold is grown mid-doubling without changing a
+result; an in-place doubling needs less HBM beside the table than an
+out-of-place one (include/Streaming_Kmer_Hash_Table.hpp:1473-1681 doubles its
+store in place too)."""
+import numpy as np
+import pytest
+
+import kache_hash_amd as KA
+import oracle as O
+import seqgen as G
+from kache_hash_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    return KA.api()
+
+
+def run(api, width, reads, queries, mode, away=0, expected=20_000, users=2, op=None, x=1):
+    """Inserts `reads` in calls of 2,000 by alternating users; returns the
+    per-call returns, capacities after each call, dump, lookups and info."""
+    op = op if op is not None else (capi.ADD if width else capi.KEEP)
+    t = capi.Table(api, width, expected_keys=expected)
+    t.set_growth(mode, away)
+    us = [t.register() for _ in range(users)]
+    rets, caps = [], []
+    for i in range(0, len(reads), 2000):
+        rets.append(t.insert(reads[i:i + 2000], op, x, us[(i // 2000) % users]))
+        caps.append(t.stats()["capacity"])
+    k, v, _ = t.for_each()
+    look = t.lookup(queries)
+    counts = t.count(queries)
+    info = t.info()
+    t.close()
+    return rets, caps, (k, v), look, counts, info
+
+
+def oracle_run(width, reads, queries, expected=20_000, users=2, op=None, x=1):
+    op = op if op is not None else (capi.ADD if width else capi.KEEP)
+    ot, sched = O.OTable(width), O.Sched(expected)
+    rets, caps = [], []
+    for i in range(0, len(reads), 2000):
+        r = ot.insert(reads[i:i + 2000], op, x)
+        sched.call((i // 2000) % users, r)
+        rets.append(r)
+        caps.append(sched.cap * 32)
+    return rets, caps, ot.dump(), ot.lookup(queries), ot.counts(queries)[0]
+
+
+def same(a, b, width):
+    assert a[0] == b[0], "per-call returns"
+    assert a[1] == b[1], "capacities"
+    assert np.array_equal(a[2][0], b[2][0]) and np.array_equal(a[2][1], b[2][1]), "dump"
+    f1, v1, w1 = a[3]
+    f2, v2, w2 = b[3]
+    assert np.array_equal(f1, f2) and np.array_equal(w1, w2), "lookups"
+    if width:
+        assert np.array_equal(v1, v2), "lookup values"
+    assert np.array_equal(a[4], b[4]), "counts"
+
+
+@pytest.mark.parametrize("width", [0, 1, 4, 8])
+def test_in_place_doublings_match_the_oracle(api, width):
+    g = G.genome(600_000, 301 + width)
+    reads = G.reads(g, 40_000, 150, 302 + width, sub=0.01, nrate=0.001, lower=0.2)
+    groups = [s for grp in G.minimizer_groups(4, 150, 303 + width) for s in grp]
+    reads = reads[:20_000] + groups + reads[20_000:]
+    queries = G.reads(g, 600, 150, 304, sub=0.01) + groups[::7] + G.pieces(G.genome(4_000, 305), 300)
+    print(f"in place, width {width}: {len(reads)} sequences from a 20k-key table", flush=True)
+    got = run(api, width, reads, queries, capi.GROW_IN_PLACE)
+    info = got[5]
+    assert info["resizes"] >= 4 and info["in_place_doublings"] >= 4, info
+    # A range reserves room for four doublings; when that is used up the next
+    # doubling moves the store, out of place, into a range with room again.
+    assert info["store_mapped_bytes"] >= info["buckets"] * info["bucket_bytes"]
+    assert info["store_reserved_bytes"] >= info["store_mapped_bytes"]
+    same(got, oracle_run(width, reads, queries), width)
+
+
+@pytest.mark.parametrize("width", [0, 8])
+def test_in_place_equals_out_of_place(api, width):
+    g = G.genome(800_000, 311 + width)
+    reads = G.reads(g, 40_000, 150, 312, sub=0.01, nrate=0.001, lower=0.1)
+    queries = G.reads(g, 500, 150, 313, sub=0.01)
+    print(f"in place vs out of place, width {width}", flush=True)
+    a = run(api, width, reads, queries, capi.GROW_IN_PLACE)
+    b = run(api, width, reads, queries, capi.GROW_OUT_OF_PLACE)
+    # The first doubling moves the small (hipMalloc'd) store into a range that
+    # can grow; a range's reservation holds a few doublings, then the store
+    # moves again, out of place.
+    assert a[5]["in_place_doublings"] >= max(3, a[5]["resizes"] // 2)
+    assert b[5]["in_place_doublings"] == 0 and b[5]["resizes"] == a[5]["resizes"]
+    same(a, b, width)
+
+
+@pytest.mark.parametrize("width", [0, 4, 8])
+def test_away_list_grows_mid_doubling(api, width):
+    """An away list of 256 entries (one per region) fills in every doubling;
+    each full phase is run again on a grown list, and every result stays
+    exact."""
+    g = G.genome(500_000, 321 + width)
+    reads = G.reads(g, 30_000, 150, 322, sub=0.02, nrate=0.001, lower=0.1)
+    groups = [s for grp in G.minimizer_groups(6, 140, 323 + width) for s in grp]
+    reads = groups + reads
+    queries = G.reads(g, 400, 150, 324, sub=0.02) + groups[::5]
+    print(f"away list regrowth, width {width}", flush=True)
+    got = run(api, width, reads, queries, capi.GROW_IN_PLACE, away=256, expected=8_000)
+    assert got[5]["in_place_doublings"] >= 4 and got[5]["away_list_regrows"] >= 1, got[5]
+    same(got, oracle_run(width, reads, queries, expected=8_000), width)
+
+
+def test_in_place_oriented_and_values(api):
+    """Oriented OR inserts (u8) and saturating adds (u16) through in-place doublings."""
+    g = G.genome(400_000, 331)
+    reads = G.reads(g, 24_000, 150, 332, sub=0.01, nrate=0.001)
+    queries = G.reads(g, 300, 150, 333)
+    print("in place: oriented OR u8, saturating ADD u16", flush=True)
+    t = capi.Table(api, 1, expected_keys=10_000)
+    t.set_growth(capi.GROW_IN_PLACE)
+    ot = O.OTable(1)
+    u = t.register()
+    for i in range(0, len(reads), 3000):
+        part = reads[i:i + 3000]
+        assert t.insert_oriented(part, capi.OR, 1, 2, u) == ot.insert_oriented(part, capi.OR, 1, 2)
+    k, v, _ = t.for_each()
+    ko, vo = ot.dump()
+    assert np.array_equal(k, ko) and np.array_equal(v, vo)
+    assert t.info()["in_place_doublings"] >= 3
+    t.close()
+    t = capi.Table(api, 2, expected_keys=10_000)
+    t.set_growth(capi.GROW_IN_PLACE)
+    ot = O.OTable(2)
+    u = t.register()
+    for i in range(0, len(reads), 3000):
+        part = reads[i:i + 3000]
+        assert t.insert(part, capi.ADD, 9000, u) == ot.insert(part, capi.ADD, 9000)
+    k, v, _ = t.for_each()
+    ko, vo = ot.dump()
+    assert np.array_equal(k, ko) and np.array_equal(v, vo)
+    f1, v1, w1 = t.lookup(queries)
+    f2, v2, w2 = ot.lookup(queries)
+    assert (f1 == f2).all() and (v1 == v2).all() and (w1 == w2).all()
+    t.close()
+
+
+def test_in_place_needs_less_memory_and_auto_mode(api):
+    """A 3M-key u64 table (a store of ~54 MB) doubles in place by default;
+    its next doubling needs about half the HBM an out-of-place one would."""
+    g = G.genome(3_000_000, 341)
+    reads = G.reads(g, 60_000, 150, 342, sub=0.005, nrate=0.0005, lower=0.1)
+    queries = G.reads(g, 1_000, 150, 343, sub=0.005)
+    print("auto mode on a 3M-key u64 map", flush=True)
+    t = capi.Table(api, 8, expected_keys=3_000_000)
+    info = t.info()
+    assert info["growth_mode"] == capi.GROW_AUTO and info["store_reserved_bytes"] >= 2 * info["buckets"] * info["bucket_bytes"]
+    in_place_need = info["next_doubling_bytes"]
+    t.set_growth(capi.GROW_OUT_OF_PLACE)
+    out_need = t.info()["next_doubling_bytes"]
+    assert in_place_need < 0.6 * out_need, (in_place_need, out_need)
+    t.set_growth(capi.GROW_AUTO)
+    ot, sched = O.OTable(8), O.Sched(3_000_000)
+    u = t.register()
+    for i in range(0, len(reads), 20_000):
+        part = reads[i:i + 20_000]
+        r = t.insert(part, capi.ADD, 1, u)
+        assert r == ot.insert(part, capi.ADD, 1)
+        sched.call(0, r)
+    assert t.stats()["capacity"] == sched.cap * 32
+    assert t.info()["in_place_doublings"] == t.info()["resizes"] >= 1
+    k, v, _ = t.for_each()
+    ko, vo = ot.dump()
+    assert np.array_equal(k, ko) and np.array_equal(v, vo)
+    assert (t.count(queries) == ot.counts(queries)[0]).all()
+    t.close()
