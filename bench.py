@@ -359,7 +359,7 @@ def main():
         "timed_kmers": timed_kmers, "kmers_per_step": kmers_per_pass,
         "keys": stats["keys"], "capacity": stats["capacity"], "resizes": info["resizes"],
         "in_place_doublings": info["in_place_doublings"], "store_mapped_bytes": info["store_mapped_bytes"],
-        "last_doubling_away_keys": info["last_away_keys"],
+        "last_doubling_away_keys": info["last_away_keys"], "away_list_regrows": info["away_list_regrows"],
         "placement_pct": [stats["primary_pct"], stats["secondary_pct"], stats["overflow_pct"]],
         "kernel_ms": {"insert": ins_ms, "split": split_ms, "reinsert": re_ms, "query": q_ms,
                       "insert_launches": ins_launches, "query_launches": q_launches, "profiled_steps": profiled},

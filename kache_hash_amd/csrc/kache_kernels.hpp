@@ -647,6 +647,32 @@ __global__ void k_away_snap(const uint64_t* cursor, uint32_t regions, uint64_t* 
         snap[r] = cursor[uint64_t(r) * AWAY_CURSOR_STRIDE];
 }
 
+// Moves the away list into one with larger regions: region r's first
+// before[r] entries (its cursor before the phase that found it full), then
+// resets the cursors to those counts and clears the fail word, so that the
+// phase can run again on the new list.
+__global__ void k_away_regrow(AwayList from, AwayList to, const uint64_t* before)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for(uint32_t r = 0; r < from.regions; ++r)
+    {
+        const uint64_t n = before[r];
+        for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += stride)
+        {
+            to.keys[r * to.region + i] = from.keys[r * from.region + i];
+            if(from.vals)
+                to.vals[r * to.region + i] = from.vals[r * from.region + i];
+        }
+    }
+    if(blockIdx.x == 0)
+    {
+        for(uint32_t r = threadIdx.x; r < from.regions; r += blockDim.x)
+            to.cursor[uint64_t(r) * AWAY_CURSOR_STRIDE] = before[r];
+        if(threadIdx.x == 0)
+            *to.fail = 0;
+    }
+}
+
 // The keys of the away list and of the old overflow table into the doubled
 // table, with the insertion policy and the reinsert's claims (insert_new_wave).
 // Each key inserted is cleared from its list entry or old overflow slot, so

@@ -208,6 +208,15 @@ private:
     std::map<const void*, Range> ranges;
     std::map<int, int> support;
 
+public:
+    // A physical chunk not yet mapped.
+    struct Chunk
+    {
+        hipMemGenericAllocationHandle_t h;
+        uint64_t n;
+    };
+
+private:
     static hipMemAllocationProp prop(int dev)
     {
         hipMemAllocationProp p = {};
@@ -222,7 +231,7 @@ private:
     // chunk, or to the new part alone, is refused with "invalid argument"
     // once the range holds chunks of different sizes).  On failure the range
     // is left as it was and the error thrown.
-    static void map_to(void* base, Range& r, uint64_t bytes)
+    static void map_to(void* base, Range& r, uint64_t bytes, std::vector<Chunk>* ready = nullptr)
     {
         const hipMemAllocationProp pr = prop(r.dev);
         hipMemAccessDesc acc = {};
@@ -236,7 +245,10 @@ private:
             {
                 const uint64_t n = std::min(CHUNK, bytes - r.mapped);
                 hipMemGenericAllocationHandle_t h;
-                check(hipMemCreate(&h, n, &pr, 0), "hipMemCreate");
+                if(ready && !ready->empty() && ready->front().n == n)
+                    h = ready->front().h, ready->erase(ready->begin());
+                else
+                    check(hipMemCreate(&h, n, &pr, 0), "hipMemCreate");
                 const hipError_t e = hipMemMap(static_cast<char*>(base) + r.mapped, n, 0, h, 0);
                 if(e != hipSuccess)
                 {
@@ -349,8 +361,42 @@ public:
         return it == ranges.end() ? 0 : it->second.mapped;
     }
 
-    // Maps the range at `p` up to `bytes` (rounded; at most its reservation).
-    void grow(void* p, uint64_t bytes)
+    // Physical chunks for mapping a range from `from` to `to` bytes, in the
+    // sizes map_to takes them: creating them is what costs (the driver clears
+    // memory it hands out), so a helper thread makes them ahead of a doubling.
+    static std::vector<Chunk> make_chunks(int dev, uint64_t from, uint64_t to)
+    {
+        const hipMemAllocationProp pr = prop(dev);
+        std::vector<Chunk> out;
+        try
+        {
+            for(uint64_t at = from; at < to;)
+            {
+                const uint64_t n = std::min(CHUNK, to - at);
+                hipMemGenericAllocationHandle_t h;
+                check(hipMemCreate(&h, n, &pr, 0), "hipMemCreate");
+                out.push_back({h, n});
+                at += n;
+            }
+        }
+        catch(...)
+        {
+            release_chunks(out);
+            throw;
+        }
+        return out;
+    }
+
+    static void release_chunks(std::vector<Chunk>& c)
+    {
+        for(const Chunk& k : c)
+            (void)hipMemRelease(k.h);
+        c.clear();
+    }
+
+    // Maps the range at `p` up to `bytes` (rounded; at most its reservation),
+    // with chunks from `ready` where they fit (those used are taken out).
+    void grow(void* p, uint64_t bytes, std::vector<Chunk>* ready = nullptr)
     {
         std::unique_lock<std::mutex> lk(mu);
         const auto it = ranges.find(p);
@@ -360,7 +406,7 @@ public:
         lk.unlock();
         // Only the range's owner maps it: nothing else changes it meanwhile.
         // A failure leaves the range as it was.
-        map_to(p, r, rounded(bytes));
+        map_to(p, r, rounded(bytes), ready);
         lk.lock();
         ranges[p] = std::move(r);
     }
@@ -849,7 +895,7 @@ struct kache_table
     {
         wait_frees();
         if(prepared.valid())
-            free_prepared(prepared.get());
+            { Prepared p = prepared.get(); free_prepared(p); }
         if(stream)
             (void)hipStreamSynchronize(stream);
         free_graphs();
@@ -1214,10 +1260,11 @@ struct kache_table
     struct Prepared
     {
         uint64_t cap = 0, ov_slots = 0, bucket_b = 0, ov_b = 0, away_bytes = 0;
-        bool in_place = false;      // only the overflow table is prepared
+        bool in_place = false;      // only the overflow table and the store's new chunks are prepared
         void* buckets = nullptr;
         void* ov = nullptr;
         void* away = nullptr;
+        std::vector<VmSpace::Chunk> chunks;     // in place: physical memory for the store's second half
     };
     std::future<Prepared> prepared;
 
@@ -1237,12 +1284,16 @@ struct kache_table
             return;     // the doubling would be refused: nothing to prepare
         const bool store_range = !p.in_place && growable(p.bucket_b);
         const uint64_t reserve = store_range ? growth_reserve(p.bucket_b) : 0;
-        prepared = std::async(std::launch::async, [p, reserve, store_range, dev = device]() mutable
+        const uint64_t mapped = p.in_place ? VmSpace::get().mapped(buckets) : 0;
+        const uint64_t target = VmSpace::rounded(p.cap * bucket_bytes());
+        prepared = std::async(std::launch::async, [p, reserve, store_range, mapped, target, dev = device]() mutable
         {
             (void)hipSetDevice(dev);
             try
             {
                 p.ov = ArrayCache::get().take(dev, p.ov_b);
+                if(p.in_place && mapped < target)
+                    p.chunks = VmSpace::make_chunks(dev, mapped, target);
                 if(!p.in_place)
                 {
                     p.buckets = store_range ? ArrayCache::get().take_range(dev, p.bucket_b, reserve) : ArrayCache::get().take(dev, p.bucket_b);
@@ -1256,8 +1307,9 @@ struct kache_table
         });
     }
 
-    void free_prepared(const Prepared& p) const
+    void free_prepared(Prepared& p) const
     {
+        VmSpace::release_chunks(p.chunks);
         ArrayCache::get().give(device, p.buckets, p.bucket_b);
         ArrayCache::get().give(device, p.ov, p.ov_b);
         ArrayCache::get().give(device, p.away, p.away_bytes);
@@ -1403,7 +1455,7 @@ struct kache_table
         const uint64_t extra = in_place_extra();
         check_budget(cap * bb + ov_bytes(nov_slots) + extra, "doubling the table",
                      (2 * cap * bb > mapped ? 2 * cap * bb - mapped : 0) + ov_bytes(nov_slots) + extra);
-        const Prepared pre = take_prepared(ncap, nov_slots, true);
+        Prepared pre = take_prepared(ncap, nov_slots, true);
 
         uint64_t* nov = static_cast<uint64_t*>(pre.ov);
         uint8_t* copy = nullptr;
@@ -1414,6 +1466,7 @@ struct kache_table
         const auto give_new = [&]
         {
             (void)hipStreamSynchronize(stream);
+            VmSpace::release_chunks(pre.chunks);
             ArrayCache::get().give(device, nov, ov_bytes(nov_slots)), nov = nullptr;
             ArrayCache::get().give(device, copy, K * bb), copy = nullptr;
             ArrayCache::get().give(device, ctl, AWAY_CTL_WORDS * 8), ctl = nullptr;
@@ -1422,7 +1475,8 @@ struct kache_table
         };
         try
         {
-            VmSpace::get().grow(buckets, ncap * bb);
+            VmSpace::get().grow(buckets, ncap * bb, &pre.chunks);
+            VmSpace::release_chunks(pre.chunks);     // any not used
             if(nov)
                 zero(nov, ov_bytes(nov_slots));
             else
@@ -1488,25 +1542,19 @@ struct kache_table
             // Phase fail - 1 found its region full: grow the list, keep each
             // region's entries from before that phase, and run it again.
             const size_t k = fail - 1;
-            std::vector<uint64_t> before(regions), cur(AWAY_REGIONS * AWAY_CURSOR_STRIDE + 8, 0);
-            HIP(hipMemcpy(before.data(), snaps + k * AWAY_REGIONS, regions * 8, hipMemcpyDeviceToHost));
             const uint64_t region2 = 2 * L.region, entries2 = region2 * regions;
-            uint64_t* k2 = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
-            uint64_t* v2 = vw ? static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8)) : nullptr;
-            for(uint32_t r = 0; r < regions; ++r)
-                if(before[r])
-                {
-                    HIP(hipMemcpyAsync(k2 + r * region2, lkeys + r * L.region, before[r] * 8, hipMemcpyDeviceToDevice, stream));
-                    if(vw)
-                        HIP(hipMemcpyAsync(v2 + r * region2, lvals + r * L.region, before[r] * 8, hipMemcpyDeviceToDevice, stream));
-                    cur[r * AWAY_CURSOR_STRIDE] = before[r];
-                }
-            HIP(hipMemcpyAsync(ctl, cur.data(), cur.size() * 8, hipMemcpyHostToDevice, stream));     // cursors, and fail = 0
+            AwayList L2 = L;
+            L2.keys = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
+            L2.vals = vw ? static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8)) : nullptr;
+            L2.region = region2;
+            hipLaunchKernelGGL(k_away_regrow, dim3(std::min<uint64_t>(1024, (L.region + 255) / 256 + 1)), dim3(256), 0, stream, L, L2,
+                               static_cast<const uint64_t*>(snaps + k * AWAY_REGIONS));
+            HIP(hipGetLastError());
             sync();
             ArrayCache::get().give(device, lkeys, entries * 8);
             ArrayCache::get().give(device, lvals, entries * 8);
-            lkeys = k2, lvals = v2, entries = entries2;
-            L.keys = lkeys, L.vals = lvals, L.region = region2;
+            lkeys = L2.keys, lvals = L2.vals, entries = entries2;
+            L = L2;
             away_regrows++;
             run_phases(k);
         }
@@ -2979,7 +3027,7 @@ struct kache_table
         // Arrays prepared for the old table's next doubling are of no use now.
         wait_frees();
         if(prepared.valid())
-            free_prepared(prepared.get());
+            { Prepared p = prepared.get(); free_prepared(p); }
         uint8_t* nb = take_buckets(h.cap);
         uint64_t* nov = nullptr;
         try
