@@ -621,6 +621,7 @@ def serve_main(a, KA, capi, api):
     api.check(api.lib.kache_amd_synth_genome(g.ptr, n, seed), "synth_genome")
     api.check(api.lib.kache_amd_device_fill(g.ptr + n, ord("\n"), 1 + capi.TEXT_PAD), "fill")
     t = KA.table(width=4, expected_keys=a.serve_expected)
+    t.set_growth(GROWTH[a.growth])
     u = t.register()
     t.insert_device(g.ptr, n + 1, KA.ADD, 1, u)
     genome = g.download(n)
@@ -807,6 +808,7 @@ def counter_main(a, KA, capi, api):
     print(f"counter: {len(prime) >> 20} MiB of {n_segs} segments, {len(reads) >> 20} MiB of reads in host memory "
           f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     t = KA.table(width=4, expected_keys=a.counter_genome, load_factor=0.9, pow2=True)
+    t.set_growth(GROWTH[a.growth])
     u = t.register()
     t.profile(True)
     t1 = time.perf_counter()
@@ -960,6 +962,7 @@ def batched_main(a, KA, capi, api):
     results = {}
     for name, width, oriented in (("u32_count", 4, False), ("u8_oriented", 1, True)):
         t = KA.table(width=width, expected_keys=a.batched_expected)
+        t.set_growth(GROWTH[a.growth])
         u = t.register()
         t.autotune(2)
         t.profile(True)

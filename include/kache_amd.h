@@ -174,7 +174,10 @@ int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
  * the keys away from their primary bucket (16 bytes each) beside the table,
  * instead of a second, doubled store.  Out of place: a new store of twice the
  * size, the old one freed after.  Results are the same either way.
- *   KACHE_AMD_GROW_AUTO          in place for stores of 2 MiB and more (the default)
+ *   KACHE_AMD_GROW_AUTO          (the default) stores of 2 MiB and more can grow in place;
+ *                                a doubling runs in place when an out-of-place one would not
+ *                                fit the table's budget or would leave less than 1/16 of the
+ *                                device's memory free, out of place otherwise (2-3% faster)
  *   KACHE_AMD_GROW_IN_PLACE      in place at every size; a store that cannot
  *                                grow in place moves out of place once, into one that can
  *   KACHE_AMD_GROW_OUT_OF_PLACE  always out of place

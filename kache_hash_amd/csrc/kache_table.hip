@@ -1150,11 +1150,32 @@ struct kache_table
         return std::max(VmSpace::rounded(bytes), std::min(16 * VmSpace::rounded(bytes), top));
     }
 
-    // Whether the next doubling runs in place: the store is a range whose
-    // reservation holds twice its buckets.
+    // HBM an out-of-place doubling needs beside the current arrays: the
+    // doubled store, its overflow table and the split's bitmap.
+    uint64_t out_of_place_bytes() const
+    {
+        return 2 * cap * bucket_bytes() + ov_bytes(next_ov_slots()) + std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
+    }
+
+    // Whether the next doubling runs in place.  It can when the store is a
+    // range whose reservation holds twice its buckets.  Automatically it
+    // does only when an out-of-place doubling would not fit: beyond the
+    // table's budget, or leaving less than 1/16 of the device's memory free
+    // (out of place measured 2-3% faster on the batched workload: one split
+    // launch instead of eight phases, no mapping).
     bool in_place_next() const
     {
-        return growth_mode != KACHE_AMD_GROW_OUT_OF_PLACE && VmSpace::get().reserved(buckets) >= VmSpace::rounded(2 * cap * bucket_bytes());
+        if(growth_mode == KACHE_AMD_GROW_OUT_OF_PLACE || VmSpace::get().reserved(buckets) < VmSpace::rounded(2 * cap * bucket_bytes()))
+            return false;
+        if(growth_mode == KACHE_AMD_GROW_IN_PLACE)
+            return true;
+        const uint64_t out = out_of_place_bytes();
+        if(hbm_budget != 0 && device_bytes() + out > hbm_budget)
+            return true;
+        size_t free_b = 0, total_b = 0;
+        if(hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return true;
+        return out + total_b / 16 > free_b + ArrayCache::get().bytes_kept(device);
     }
 
     // Share of the keys the last in-place doubling listed (keys away from
@@ -1185,7 +1206,7 @@ struct kache_table
     {
         if(in_place_next())
             return cap * bucket_bytes() + ov_bytes(next_ov_slots()) + in_place_extra();
-        return 2 * cap * bucket_bytes() + ov_bytes(next_ov_slots()) + std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
+        return out_of_place_bytes();
     }
 
     // Throws bad_alloc (KACHE_ENOMEM) before growth that would need `more`

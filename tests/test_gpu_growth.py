@@ -41,10 +41,13 @@ def test_budget_refuses_a_doubling_and_the_table_stays_exact(api, width):
     ot = O.OTable(width)
     u = t.register()
     cap0 = t.stats()["capacity"]
-    # Room for overflow growth (<= 16 MB), not for a doubling: half what the
-    # next doubling needs beside the table (in place: the store's second half,
-    # >= 25 MB, and the new overflow table).
+    # Room for overflow growth (<= 16 MB), not for a doubling: half what an
+    # in-place doubling needs beside the table (the store's second half,
+    # >= 25 MB, and the new overflow table; out of place needs twice that,
+    # so under this budget the table would double in place).
+    t.set_growth(capi.GROW_IN_PLACE)
     info = t.info()
+    t.set_growth(capi.GROW_AUTO)
     t.set_hbm_budget(info["device_bytes"] + info["next_doubling_bytes"] // 2)
     refused = None
     for i in range(0, len(reads), 4000):

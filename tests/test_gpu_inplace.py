@@ -150,30 +150,48 @@ def test_in_place_oriented_and_values(api):
     t.close()
 
 
-def test_in_place_needs_less_memory_and_auto_mode(api):
-    """A 3M-key u64 table (a store of ~54 MB) doubles in place by default;
-    its next doubling needs about half the HBM an out-of-place one would."""
+def test_auto_mode_doubles_in_place_when_out_of_place_does_not_fit(api):
+    """A 3M-key u64 table (a store of ~54 MB).  In place, its next doubling
+    needs about half the HBM an out-of-place one needs.  Automatically it
+    doubles out of place while the device has room, and in place once a
+    budget admits the in-place doubling but not the out-of-place one; every
+    result stays equal to the oracle's."""
     g = G.genome(3_000_000, 341)
-    reads = G.reads(g, 60_000, 150, 342, sub=0.005, nrate=0.0005, lower=0.1)
+    reads = G.reads(g, 240_000, 150, 342, sub=0.005, nrate=0.0005, lower=0.1)
     queries = G.reads(g, 1_000, 150, 343, sub=0.005)
     print("auto mode on a 3M-key u64 map", flush=True)
     t = capi.Table(api, 8, expected_keys=3_000_000)
     info = t.info()
     assert info["growth_mode"] == capi.GROW_AUTO and info["store_reserved_bytes"] >= 2 * info["buckets"] * info["bucket_bytes"]
-    in_place_need = info["next_doubling_bytes"]
+    t.set_growth(capi.GROW_IN_PLACE)
+    in_need = t.info()["next_doubling_bytes"]
     t.set_growth(capi.GROW_OUT_OF_PLACE)
     out_need = t.info()["next_doubling_bytes"]
-    assert in_place_need < 0.6 * out_need, (in_place_need, out_need)
+    assert in_need < 0.6 * out_need, (in_need, out_need)
     t.set_growth(capi.GROW_AUTO)
+    assert t.info()["next_doubling_bytes"] == out_need     # the device has room: out of place
     ot, sched = O.OTable(8), O.Sched(3_000_000)
     u = t.register()
-    for i in range(0, len(reads), 20_000):
-        part = reads[i:i + 20_000]
+    doubled_in_place = False
+    for i in range(0, len(reads), 5_000):
+        part = reads[i:i + 5_000]
+        if t.info()["resizes"] == 1 and not doubled_in_place:
+            # Budget for the next doubling: in place fits, out of place does
+            # not (half way between what the two need).
+            inf = t.info()
+            t.set_growth(capi.GROW_IN_PLACE)
+            need_in = t.info()["next_doubling_bytes"]
+            t.set_growth(capi.GROW_OUT_OF_PLACE)
+            need_out = t.info()["next_doubling_bytes"]
+            t.set_growth(capi.GROW_AUTO)
+            t.set_hbm_budget(inf["device_bytes"] + (need_in + need_out) // 2)
+            doubled_in_place = True
         r = t.insert(part, capi.ADD, 1, u)
         assert r == ot.insert(part, capi.ADD, 1)
         sched.call(0, r)
+    inf = t.info()
     assert t.stats()["capacity"] == sched.cap * 32
-    assert t.info()["in_place_doublings"] == t.info()["resizes"] >= 1
+    assert inf["resizes"] >= 2 and inf["in_place_doublings"] >= 1 and inf["resizes"] - inf["in_place_doublings"] >= 1, inf
     k, v, _ = t.for_each()
     ko, vo = ot.dump()
     assert np.array_equal(k, ko) and np.array_equal(v, vo)
