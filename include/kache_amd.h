@@ -214,15 +214,16 @@ typedef struct kache_amd_tuning
     uint32_t graph;
     uint32_t tuning;            /* get: rounds left to the tuner, 0 when done */
     uint32_t waves_per_simd;    /* occupancy shape of the insert kernel: 6, 7 or 8 (0 on set: the default, 7) */
+    uint32_t threads_per_block; /* its tile: threads per block = positions per tile, 64, 128 or 256 (0 on set: 256) */
 } kache_amd_tuning;
 int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) KACHE_NOEXCEPT;
 int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) KACHE_NOEXCEPT;
 
 /* Online autotuner of the insert launches of device-text calls: a
  * coordinate descent over the launch shape, one dimension per stage in the
- * order waves per SIMD the insert kernel is built for (6, 7, 8), blocks per
- * CU (all that fit, 3/4, 1/2), positions per launch (2^23, 2^24, 2^25) and
- * graph replay (off, on).  A stage's launches alternate the best shape so far
+ * order waves per SIMD the insert kernel is built for (6, 7, 8), its tile
+ * (64, 128, 256 threads per block), blocks per CU (all that fit, 3/4, 1/2),
+ * positions per launch (2^23, 2^24, 2^25) and graph replay (off, on).  A stage's launches alternate the best shape so far
  * (the reference) with each candidate, `rounds` times; a candidate's score
  * is the median ratio of its host-timed time per position to the mean of the
  * reference launches either side of it (which cancels the drift of the

@@ -49,7 +49,7 @@ REQ_COUNT, REQ_LOOKUP, REQ_UPDATE, REQ_INSERT = range(4)   # KACHE_AMD_REQ_*
 
 class Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("launch_log2", C.c_uint32), ("graph", C.c_uint32), ("tuning", C.c_uint32),
-                ("waves_per_simd", C.c_uint32)]
+                ("waves_per_simd", C.c_uint32), ("threads_per_block", C.c_uint32)]
 
 
 class TuningTrial(C.Structure):
@@ -456,8 +456,8 @@ class Table:
         self.api.check(self.api.lib.kache_amd_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)), "kache_amd_kernel_time")
         return ms.value, n.value
 
-    def set_tuning(self, blocks_per_cu=0, launch_log2=24, graph=False, waves_per_simd=0):
-        cfg = Tuning(blocks_per_cu, launch_log2, int(graph), 0, waves_per_simd)
+    def set_tuning(self, blocks_per_cu=0, launch_log2=24, graph=False, waves_per_simd=0, threads_per_block=0):
+        cfg = Tuning(blocks_per_cu, launch_log2, int(graph), 0, waves_per_simd, threads_per_block)
         self.api.check(self.api.lib.kache_amd_set_tuning(self.h, C.byref(cfg)), "kache_amd_set_tuning")
 
     def tuning(self):
@@ -474,7 +474,7 @@ class Table:
         out = (TuningTrial * max(n.value, 1))()
         self.api.check(self.api.lib.kache_amd_tuning_trials(self.h, out, n.value, C.byref(n)), "kache_amd_tuning_trials")
         return [{"blocks_per_cu": t.cfg.blocks_per_cu, "launch_log2": t.cfg.launch_log2, "graph": t.cfg.graph,
-                 "waves_per_simd": t.cfg.waves_per_simd, "launches": t.launches,
+                 "waves_per_simd": t.cfg.waves_per_simd, "threads_per_block": t.cfg.threads_per_block, "launches": t.launches,
                  "median_ns_per_position": t.median_ns_per_position,
                  "median_vs_reference": t.median_vs_reference} for t in out[:n.value]]
 

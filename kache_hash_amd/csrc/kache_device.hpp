@@ -890,19 +890,23 @@ __device__ __forceinline__ Desc describe_wave(const uint8_t* text, uint64_t p0, 
 // position origin + t * TILE + threadIdx.x) and origin is p_begin rounded
 // down to 16 bytes (tiles load aligned words); positions before p_begin are
 // not described.  `s` is the wave's own LDS.
-template <typename F>
+// BT: positions per tile, the block's threads (64, 128 or 256: a tile's
+// last wave reads up to BT + 64 bytes past the tile's start, within the 512
+// readable bytes a device text has past its end).
+template <int BT = TILE, typename F>
 __device__ __forceinline__ void for_tiles(const uint8_t* text, uint64_t p_begin, uint64_t pos_end, uint64_t text_len, WaveLds& s, const F& f)
 {
+    static_assert(BT == 64 || BT == 128 || BT == 256, "tiles of 1, 2 or 4 waves");
     const uint64_t origin = p_begin & ~uint64_t(15);
-    const uint64_t tiles = (pos_end - origin + TILE - 1) / TILE;
+    const uint64_t tiles = (pos_end - origin + BT - 1) / BT;
     uint2 w{};
     if(blockIdx.x < tiles)
-        load_wave_tile(text, origin + blockIdx.x * uint64_t(TILE), w);
+        load_wave_tile(text, origin + blockIdx.x * uint64_t(BT), w);
     for(uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
         const uint64_t next = tile + gridDim.x;
-        Desc d = describe_wave(text, origin + tile * TILE, pos_end, text_len, s, w, origin + next * TILE, next < tiles);
-        d.valid &= (origin + tile * TILE + threadIdx.x >= p_begin);
+        Desc d = describe_wave(text, origin + tile * BT, pos_end, text_len, s, w, origin + next * BT, next < tiles);
+        d.valid &= (origin + tile * BT + threadIdx.x >= p_begin);
         f(tile, d);
     }
 }

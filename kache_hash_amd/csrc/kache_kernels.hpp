@@ -77,15 +77,15 @@ __device__ __forceinline__ void hand_off(const Handoff& h)
     }
 }
 
-// WAVES: the waves per SIMD its registers must allow (the occupancy shape
-// the launch tuner chooses among).
-template <int VB, uint32_t OP, bool ORIENT, int WAVES = KAMD_INSERT_WAVES>
-__global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
+// WAVES: the waves per SIMD its registers must allow; BT: the block's
+// threads, i.e. positions per tile (the launch shape the tuner chooses).
+template <int VB, uint32_t OP, bool ORIENT, int WAVES = KAMD_INSERT_WAVES, int BT = TILE>
+__global__ __launch_bounds__(BT, WAVES) void k_insert(TableView T, const uint8_t* __restrict__ text, uint64_t text_len, uint64_t pos_begin, uint64_t pos_end, uint64_t x, uint64_t x_rev, Counters* cnt, Handoff prev)
 {
     hand_off(prev);
-    __shared__ WaveLds wave_lds[TILE / 64];
-    __shared__ uint64_t red[TILE / 64];
-    __shared__ DeferRing rings[TILE / 64];
+    __shared__ WaveLds wave_lds[BT / 64];
+    __shared__ uint64_t red[BT / 64];
+    __shared__ DeferRing rings[BT / 64];
     WaveLds& s = wave_lds[threadIdx.x >> 6];
     DeferRing& q = rings[threadIdx.x >> 6];
     const uint32_t lane = __lane_id();
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256, WAVES) void k_insert(TableView T, const uint8_
                 value_op<OP>(slot.v(), (ORIENT && !(cf >> 8)) ? x_rev : x, T.vmax, !ORIENT && r == RES_NEW);
     };
 
-    for_tiles(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
+    for_tiles<BT>(text, pos_begin, pos_end, text_len, s, [&](uint64_t, const Desc& d)
     {
         Slot<VB> slot(nullptr, 0);
         if(d.valid)

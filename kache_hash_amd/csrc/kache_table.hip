@@ -997,7 +997,7 @@ struct kache_table
         return grid_for(reinterpret_cast<const void*>(kern), work_blocks);
     }
 
-    uint32_t grid_for(const void* kern, uint64_t work_blocks) const
+    uint32_t grid_for(const void* kern, uint64_t work_blocks, uint32_t threads = TILE) const
     {
         int per_cu = static_cast<int>(blocks_per_cu);
         if(per_cu == 0)
@@ -1007,7 +1007,7 @@ struct kache_table
                 per_cu = it->second;
             else
             {
-                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TILE, 0) != hipSuccess || per_cu < 1)
+                if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, static_cast<int>(threads), 0) != hipSuccess || per_cu < 1)
                     per_cu = 4;
                 resident[kern] = per_cu;
             }
@@ -1792,46 +1792,59 @@ struct kache_table
 
     // --- Inserts ---
 
-    // The insert kernel of an operation, at `waves` waves per SIMD.
-    template <int VB, int WV>
+    // The insert kernel of an operation, at `WV` waves per SIMD and `BT`
+    // threads per block.
+    template <int VB, int WV, int BT>
     static const void* insert_kernel_of(uint32_t op, bool oriented)
     {
         const auto f = [](auto kern) { return reinterpret_cast<const void*>(kern); };
         if(oriented)
             switch(op)
             {
-            case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, true, WV>);
-            case OP_ADD:    return f(k_insert<VB, OP_ADD, true, WV>);
-            case OP_OR:     return f(k_insert<VB, OP_OR, true, WV>);
-            default:        return f(k_insert<VB, OP_MAX, true, WV>);
+            case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, true, WV, BT>);
+            case OP_ADD:    return f(k_insert<VB, OP_ADD, true, WV, BT>);
+            case OP_OR:     return f(k_insert<VB, OP_OR, true, WV, BT>);
+            default:        return f(k_insert<VB, OP_MAX, true, WV, BT>);
             }
         switch(op)
         {
-        case OP_KEEP:   return f(k_insert<VB, OP_KEEP, false, WV>);
-        case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, false, WV>);
-        case OP_ADD:    return f(k_insert<VB, OP_ADD, false, WV>);
-        case OP_OR:     return f(k_insert<VB, OP_OR, false, WV>);
-        case OP_MIN:    return f(k_insert<VB, OP_MIN, false, WV>);
-        case OP_MAX:    return f(k_insert<VB, OP_MAX, false, WV>);
-        default:        return f(k_insert<VB, OP_ADD_WRAP, false, WV>);
+        case OP_KEEP:   return f(k_insert<VB, OP_KEEP, false, WV, BT>);
+        case OP_ASSIGN: return f(k_insert<VB, OP_ASSIGN, false, WV, BT>);
+        case OP_ADD:    return f(k_insert<VB, OP_ADD, false, WV, BT>);
+        case OP_OR:     return f(k_insert<VB, OP_OR, false, WV, BT>);
+        case OP_MIN:    return f(k_insert<VB, OP_MIN, false, WV, BT>);
+        case OP_MAX:    return f(k_insert<VB, OP_MAX, false, WV, BT>);
+        default:        return f(k_insert<VB, OP_ADD_WRAP, false, WV, BT>);
         }
     }
 
-    template <int WV>
+    template <int WV, int BT>
     const void* insert_kernel_at(uint32_t op, bool oriented) const
     {
         if(!map)
-            return reinterpret_cast<const void*>(k_insert<0, OP_KEEP, false, WV>);
-        return vw == 4 ? insert_kernel_of<4, WV>(op, oriented) : insert_kernel_of<8, WV>(op, oriented);
+            return reinterpret_cast<const void*>(k_insert<0, OP_KEEP, false, WV, BT>);
+        return vw == 4 ? insert_kernel_of<4, WV, BT>(op, oriented) : insert_kernel_of<8, WV, BT>(op, oriented);
     }
 
-    const void* insert_kernel(uint32_t op, bool oriented, uint32_t waves = 0) const
+    template <int BT>
+    const void* insert_kernel_bt(uint32_t op, bool oriented, uint32_t waves) const
     {
-        switch(waves ? waves : insert_waves)
+        switch(waves)
         {
-        case 6:  return insert_kernel_at<6>(op, oriented);
-        case 8:  return insert_kernel_at<8>(op, oriented);
-        default: return insert_kernel_at<KAMD_INSERT_WAVES>(op, oriented);
+        case 6:  return insert_kernel_at<6, BT>(op, oriented);
+        case 8:  return insert_kernel_at<8, BT>(op, oriented);
+        default: return insert_kernel_at<KAMD_INSERT_WAVES, BT>(op, oriented);
+        }
+    }
+
+    const void* insert_kernel(uint32_t op, bool oriented, uint32_t waves = 0, uint32_t threads = 0) const
+    {
+        waves = waves ? waves : insert_waves;
+        switch(threads ? threads : insert_threads)
+        {
+        case 64:  return insert_kernel_bt<64>(op, oriented, waves);
+        case 128: return insert_kernel_bt<128>(op, oriented, waves);
+        default:  return insert_kernel_bt<TILE>(op, oriented, waves);
         }
     }
 
@@ -1862,7 +1875,7 @@ struct kache_table
 
     uint32_t insert_grid(const void* kern, uint64_t b, uint64_t e) const
     {
-        return grid_for(kern, (e - (b & ~uint64_t(15)) + TILE - 1) / TILE);
+        return grid_for(kern, (e - (b & ~uint64_t(15)) + insert_threads - 1) / insert_threads, insert_threads);
     }
 
     void dispatch_insert(uint32_t op, bool oriented, const uint8_t* text, uint64_t text_len, uint64_t b, uint64_t e, uint64_t x, uint64_t xr,
@@ -1871,7 +1884,7 @@ struct kache_table
         const void* kern = insert_kernel(op, oriented);
         InsertArgs a;
         insert_args(a, text, text_len, b, e, x, xr, cnt, oriented, prev);
-        HIP(hipLaunchKernel(kern, dim3(insert_grid(kern, b, e)), dim3(TILE), a.ptrs, 0, stream));
+        HIP(hipLaunchKernel(kern, dim3(insert_grid(kern, b, e)), dim3(insert_threads), a.ptrs, 0, stream));
     }
 
     // --- Launch shape and its online tuner ---
@@ -1883,6 +1896,7 @@ struct kache_table
     uint32_t launch_log2 = 24;
     bool use_graph = false;
     uint32_t insert_waves = KAMD_INSERT_WAVES;   // occupancy shape of the insert kernel (6, 7 or 8 waves per SIMD)
+    uint32_t insert_threads = TILE;              // its tile: threads per block = positions per tile (64, 128, 256)
 
     // The tuner: a coordinate descent over the shape's dimensions (waves per
     // SIMD, blocks per CU, positions per launch, graph replay).  A stage
@@ -1940,7 +1954,7 @@ struct kache_table
         if(!g.exec)
         {
             HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-            hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(TILE), a.ptrs, 0, stream);
+            hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(insert_threads), a.ptrs, 0, stream);
             if(le == hipSuccess)
             {
                 hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), reinterpret_cast<uint64_t*>(h_cnt),
@@ -1974,7 +1988,7 @@ struct kache_table
         hipKernelNodeParams kp{};
         kp.func = const_cast<void*>(kern);
         kp.gridDim = dim3(grid);
-        kp.blockDim = dim3(TILE);
+        kp.blockDim = dim3(insert_threads);
         kp.sharedMemBytes = 0;
         kp.kernelParams = a.ptrs;
         kp.extra = nullptr;
@@ -2023,6 +2037,7 @@ struct kache_table
     {
         blocks_per_cu = cfg.blocks_per_cu, use_graph = cfg.graph != 0, launch_log2 = cfg.launch_log2;
         insert_waves = cfg.waves_per_simd ? cfg.waves_per_simd : KAMD_INSERT_WAVES;
+        insert_threads = cfg.threads_per_block ? cfg.threads_per_block : TILE;
     }
 
     // The shape the next launch takes: the tuner's while it runs.
@@ -2035,7 +2050,7 @@ struct kache_table
         tune_rounds = rounds;
         if(!rounds)
             return;
-        tune_best = kache_amd_tuning{blocks_per_cu, launch_log2, use_graph ? 1u : 0u, 0, insert_waves};
+        tune_best = kache_amd_tuning{blocks_per_cu, launch_log2, use_graph ? 1u : 0u, 0, insert_waves, insert_threads};
         begin_stage();
     }
 
@@ -2053,9 +2068,19 @@ struct kache_table
                     c.waves_per_simd = wv, cands.push_back(c);
             break;
         case 1:
+            // The tile: threads per block, i.e. waves of one block walking
+            // the positions together (each wave describes and probes its 64
+            // on its own); all blocks that fit stay resident.
+            for(uint32_t bt : {64u, 128u, 256u})
+                if(bt != tune_best.threads_per_block)
+                    c.threads_per_block = bt, c.blocks_per_cu = 0, cands.push_back(c);
+            break;
+        case 2:
         {
             int resident = 0;
-            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, insert_kernel(OP_ADD, false, tune_best.waves_per_simd), TILE, 0) != hipSuccess ||
+            const uint32_t bt = tune_best.threads_per_block ? tune_best.threads_per_block : TILE;
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, insert_kernel(OP_ADD, false, tune_best.waves_per_simd, bt),
+                                                            static_cast<int>(bt), 0) != hipSuccess ||
                resident < 1)
                 resident = 8;
             for(uint32_t b : {0u, static_cast<uint32_t>(resident * 3 / 4), static_cast<uint32_t>(resident / 2)})
@@ -2063,12 +2088,12 @@ struct kache_table
                     c.blocks_per_cu = b, cands.push_back(c);
             break;
         }
-        case 2:
+        case 3:
             for(uint32_t lg : {23u, 24u, 25u})
                 if(lg != tune_best.launch_log2)
                     c.launch_log2 = lg, cands.push_back(c);
             break;
-        case 3:
+        case 4:
             c.graph = tune_best.graph ? 0u : 1u, cands.push_back(c);
             break;
         default:
@@ -3750,11 +3775,13 @@ int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) noexcept
 int kache_amd_set_tuning(kache_table* t, const kache_amd_tuning* cfg) noexcept
 {
     if(t == nullptr || cfg == nullptr || cfg->blocks_per_cu > 64 || cfg->launch_log2 < 16 || cfg->launch_log2 > 30 || cfg->graph > 1 ||
-       !(cfg->waves_per_simd == 0 || (cfg->waves_per_simd >= 6 && cfg->waves_per_simd <= 8)))
+       !(cfg->waves_per_simd == 0 || (cfg->waves_per_simd >= 6 && cfg->waves_per_simd <= 8)) ||
+       !(cfg->threads_per_block == 0 || cfg->threads_per_block == 64 || cfg->threads_per_block == 128 || cfg->threads_per_block == 256))
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
     t->blocks_per_cu = cfg->blocks_per_cu, t->launch_log2 = cfg->launch_log2, t->use_graph = cfg->graph != 0;
     t->insert_waves = cfg->waves_per_simd ? cfg->waves_per_simd : KAMD_INSERT_WAVES;
+    t->insert_threads = cfg->threads_per_block ? cfg->threads_per_block : TILE;
     t->tune_rounds = 0;
     return KACHE_OK;
 }
@@ -3764,7 +3791,7 @@ int kache_amd_get_tuning(const kache_table* t, kache_amd_tuning* cfg) noexcept
     if(t == nullptr || cfg == nullptr)
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
-    *cfg = kache_amd_tuning{t->blocks_per_cu, t->launch_log2, t->use_graph ? 1u : 0u, t->tune_rounds, t->insert_waves};
+    *cfg = kache_amd_tuning{t->blocks_per_cu, t->launch_log2, t->use_graph ? 1u : 0u, t->tune_rounds, t->insert_waves, t->insert_threads};
     return KACHE_OK;
 }
 

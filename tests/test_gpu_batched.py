@@ -87,6 +87,9 @@ SHAPES = [
     {"blocks_per_cu": 3, "launch_log2": 17, "graph": False, "waves_per_simd": 6},
     {"blocks_per_cu": 0, "launch_log2": 16, "graph": True, "waves_per_simd": 8},
     {"blocks_per_cu": 2, "launch_log2": 18, "graph": True},
+    {"blocks_per_cu": 0, "launch_log2": 16, "graph": False, "threads_per_block": 64},
+    {"blocks_per_cu": 0, "launch_log2": 17, "graph": True, "waves_per_simd": 6, "threads_per_block": 128},
+    {"blocks_per_cu": 5, "launch_log2": 16, "graph": False, "waves_per_simd": 8, "threads_per_block": 128},
     "autotune",
 ]
 
@@ -117,6 +120,8 @@ def test_launch_shapes_give_the_same_table(api, width, oriented):
         if shape == "autotune":
             trials = t.tuning_trials()
             assert len(trials) >= 3 and {tr["waves_per_simd"] for tr in trials[:3]} == {6, 7, 8}
+            if len(trials) >= 6:
+                assert {tr["threads_per_block"] for tr in trials[3:6]} == {64, 128, 256}
         t.close()
 
 
@@ -132,10 +137,13 @@ def test_tuning_validation(api):
         t.set_tuning(0, 20, False, 5)
     with pytest.raises(capi.KacheError):
         t.set_tuning(0, 20, False, 9)
+    for bad_threads in (32, 96, 512):
+        with pytest.raises(capi.KacheError):
+            t.set_tuning(0, 20, False, 7, bad_threads)
     t.set_tuning(4, 20, True)
-    assert t.tuning() == {"blocks_per_cu": 4, "launch_log2": 20, "graph": 1, "tuning": 0, "waves_per_simd": 7}
-    t.set_tuning(0, 22, False, 8)
-    assert t.tuning() == {"blocks_per_cu": 0, "launch_log2": 22, "graph": 0, "tuning": 0, "waves_per_simd": 8}
+    assert t.tuning() == {"blocks_per_cu": 4, "launch_log2": 20, "graph": 1, "tuning": 0, "waves_per_simd": 7, "threads_per_block": 256}
+    t.set_tuning(0, 22, False, 8, 128)
+    assert t.tuning() == {"blocks_per_cu": 0, "launch_log2": 22, "graph": 0, "tuning": 0, "waves_per_simd": 8, "threads_per_block": 128}
     # Batch calls check their bounds before reading them.
     u = t.register()
     d = capi.DeviceBuffer(api, 64 + capi.TEXT_PAD)
