@@ -162,7 +162,10 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
  * allocates, with KACHE_ENOMEM: the call's launches so far stay applied (as
  * a call that fails part-way in the reference), the table keeps its capacity
  * and every result stays exact, and the next flush of the growth schedule
- * tries the doubling again. */
+ * tries the doubling again.  A store taken from memory kept by earlier tables
+ * may come mapped beyond its buckets (room an in-place doubling grows into);
+ * setting a budget returns that room to the device, to within one chunk of at
+ * most 1 GiB. */
 int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
 
 /* How the table doubles (the reference doubles its store in place,

@@ -411,6 +411,27 @@ public:
         ranges[p] = std::move(r);
     }
 
+    // Unmaps the chunks of the range at `p` that lie wholly past its first
+    // `bytes`, returning their memory to the device; returns the bytes left
+    // mapped.
+    uint64_t shrink(void* p, uint64_t bytes)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        const auto it = ranges.find(p);
+        if(it == ranges.end())
+            return 0;
+        Range& r = it->second;
+        while(!r.chunks.empty() && r.mapped - r.chunks.back().second >= bytes)
+        {
+            const auto [h, n] = r.chunks.back();
+            (void)hipMemUnmap(static_cast<char*>(p) + r.mapped - n, n);
+            (void)hipMemRelease(h);
+            r.mapped -= n;
+            r.chunks.pop_back();
+        }
+        return r.mapped;
+    }
+
     void release(void* p)
     {
         Range r;
@@ -3769,6 +3790,13 @@ int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) noexcept
         return KACHE_EINVAL;
     std::lock_guard<std::mutex> lock(t->mu);
     t->hbm_budget = bytes;
+    // A store taken from kept memory may hold more than its buckets (room
+    // to grow into, already mapped); under a budget that room goes back.
+    if(bytes != 0 && t->stream)
+    {
+        (void)hipStreamSynchronize(t->stream);
+        VmSpace::get().shrink(t->buckets, t->cap * t->bucket_bytes());
+    }
     return KACHE_OK;
 }
 

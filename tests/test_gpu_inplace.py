@@ -197,3 +197,42 @@ def test_auto_mode_doubles_in_place_when_out_of_place_does_not_fit(api):
     assert np.array_equal(k, ko) and np.array_equal(v, vo)
     assert (t.count(queries) == ot.counts(queries)[0]).all()
     t.close()
+
+
+def test_budget_returns_room_a_kept_range_brought(api):
+    """A table grown in place and released leaves its range, mapped to its
+    grown size, for the next table of a similar size; a budget set on that
+    table unmaps what lies past its buckets (to within one 1-GiB chunk), and
+    the table still grows in place and stays exact."""
+    g = G.genome(1_000_000, 351)
+    reads = G.reads(g, 60_000, 150, 352, sub=0.01, nrate=0.001)
+    queries = G.reads(g, 500, 150, 353, sub=0.01)
+    print("kept range, then a budget", flush=True)
+    a = capi.Table(api, 8, expected_keys=200_000)
+    a.set_growth(capi.GROW_IN_PLACE)
+    ua = a.register()
+    for i in range(0, len(reads), 10_000):
+        a.insert(reads[i:i + 10_000], capi.ADD, 1, ua)
+    assert a.info()["in_place_doublings"] >= 2
+    a.close()
+    t = capi.Table(api, 8, expected_keys=200_000)
+    t.set_growth(capi.GROW_IN_PLACE)
+    inf = t.info()
+    logical = inf["buckets"] * inf["bucket_bytes"]
+    print(f"store: {logical} bytes of buckets, {inf['store_mapped_bytes']} mapped", flush=True)
+    assert inf["store_mapped_bytes"] >= logical
+    t.set_hbm_budget(1 << 40)
+    assert logical <= t.info()["store_mapped_bytes"] < logical + (1 << 30)
+    ot, sched = O.OTable(8), O.Sched(200_000)
+    u = t.register()
+    for i in range(0, len(reads), 10_000):
+        part = reads[i:i + 10_000]
+        r = t.insert(part, capi.ADD, 1, u)
+        assert r == ot.insert(part, capi.ADD, 1)
+        sched.call(0, r)
+    assert t.stats()["capacity"] == sched.cap * 32 and t.info()["in_place_doublings"] >= 2
+    k, v, _ = t.for_each()
+    ko, vo = ot.dump()
+    assert np.array_equal(k, ko) and np.array_equal(v, vo)
+    assert (t.count(queries) == ot.counts(queries)[0]).all()
+    t.close()
