@@ -15,8 +15,9 @@
 
 #include "kache_amd.h"
 
-// The library's packing threads (kache_table.hip).
-void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n);
+// The library's packing threads (kache_table.hip); `aside`: its second pool,
+// which a call in progress never waits for.
+void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n, bool aside);
 
 #include <algorithm>
 #include <atomic>
@@ -600,7 +601,7 @@ struct kache_amd_server
         }
         if(need > filled)
             std::memset(h + filled, '\n', need - filled);
-        kamd_copy_spans(dst.data(), src.data(), len.data(), dst.size());
+        kamd_copy_spans(dst.data(), src.data(), len.data(), dst.size(), true);
         return kache_amd_copy(d_slot[sl], h, need, 1);
     }
 

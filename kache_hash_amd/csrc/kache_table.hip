@@ -655,7 +655,7 @@ class CopyPool
         }
     }
 
-    explicit CopyPool(unsigned n)
+    explicit CopyPool(unsigned n) : threads(n + 1)
     {
         for(unsigned i = 0; i < n; ++i)
             workers.emplace_back([this]
@@ -675,6 +675,7 @@ class CopyPool
 
 public:
     static constexpr unsigned THREADS = 8;
+    const unsigned threads;     // the workers and the calling thread
 
     ~CopyPool()
     {
@@ -690,6 +691,15 @@ public:
     static CopyPool& get()
     {
         static CopyPool pool(THREADS - 1);
+        return pool;
+    }
+
+    // A second, smaller pool for work done ahead (the serving loop's stager
+    // packing the next wave), so that it never holds up the packing of a
+    // call in progress: jobs of one pool run one at a time.
+    static CopyPool& aside()
+    {
+        static CopyPool pool(3);
         return pool;
     }
 
@@ -725,7 +735,7 @@ static bool pinned_host(const void* p)
 }
 
 // Copies n bytes, in parallel when large.
-void par_copy(char* dst, const char* src, uint64_t n)
+void par_copy(char* dst, const char* src, uint64_t n, CopyPool& pool = CopyPool::get())
 {
     constexpr uint64_t PIECE = uint64_t(2) << 20;
     if(n < 2 * PIECE)
@@ -733,9 +743,9 @@ void par_copy(char* dst, const char* src, uint64_t n)
         std::memcpy(dst, src, n);
         return;
     }
-    const uint64_t parts = std::min<uint64_t>(CopyPool::THREADS, n / PIECE);
+    const uint64_t parts = std::min<uint64_t>(pool.threads, n / PIECE);
     const uint64_t each = (n + parts - 1) / parts;
-    CopyPool::get().run(parts, [&](size_t i)
+    pool.run(parts, [&](size_t i)
     {
         const uint64_t a = i * each, b = std::min(n, a + each);
         if(a < b)
@@ -3963,9 +3973,11 @@ int kache_amd_host_describe(const char* text, uint64_t text_bytes, uint64_t* key
 }
 
 // Copies n spans (each followed by a '\n' at dst[i] + len[i]), split over
-// the packing threads by bytes (other sources of the library).
-__attribute__((visibility("hidden"))) void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n)
+// the packing threads by bytes, of the second pool if `aside` (other sources
+// of the library).
+__attribute__((visibility("hidden"))) void kamd_copy_spans(char* const* dst, const char* const* src, const uint64_t* len, size_t n, bool aside)
 {
+    CopyPool& pool = aside ? CopyPool::aside() : CopyPool::get();
     uint64_t total = 0;
     for(size_t i = 0; i < n; ++i)
         total += len[i] + 1;
@@ -3982,14 +3994,14 @@ __attribute__((visibility("hidden"))) void kamd_copy_spans(char* const* dst, con
     {
         if(n == 1 && total >= (uint64_t(4) << 20))
         {
-            par_copy(dst[0], src[0], len[0]);
+            par_copy(dst[0], src[0], len[0], pool);
             dst[0][len[0]] = '\n';
         }
         else
             copy(0, n);
         return;
     }
-    const size_t parts = CopyPool::THREADS;
+    const size_t parts = pool.threads;
     std::vector<size_t> cut(parts + 1, n);
     cut[0] = 0;
     uint64_t acc = 0;
@@ -4000,7 +4012,7 @@ __attribute__((visibility("hidden"))) void kamd_copy_spans(char* const* dst, con
             acc += len[i] + 1, ++i;
         cut[p] = i;
     }
-    CopyPool::get().run(parts, [&](size_t p){ copy(cut[p], std::max(cut[p], cut[p + 1])); });
+    pool.run(parts, [&](size_t p){ copy(cut[p], std::max(cut[p], cut[p + 1])); });
 }
 
 // Sets the calling thread's last error (other sources of the library).
