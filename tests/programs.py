@@ -80,6 +80,17 @@ def same_contents(t, o):
         assert np.array_equal(v.astype(np.uint64), vo.astype(np.uint64))
 
 
+def set_growth(api, t, case):
+    """On the device library, how case `case`'s table doubles (the random
+    stream is left alone): automatically, in place at every size (every
+    other such case with an away list of 256 entries, which must grow), or
+    out of place."""
+    if not hasattr(api.lib, "kache_amd_set_growth"):
+        return
+    mode = case % 3
+    t.set_growth(mode, 256 if mode == capi.GROW_IN_PLACE and case % 2 else 0)
+
+
 def run_program(api, case):
     """Runs case `case` through a table of the library `api` (a capi.Api)
     and the oracle, asserting equality at every step."""
@@ -95,6 +106,7 @@ def run_program(api, case):
     print(f"random case {case}: width {width}, expected {expected}, pow2 {pow2}, users {users}, "
           f"genome {len(genome)}, groups {bool(groups)}, calls {calls}", flush=True)
     with capi.Table(api, width, expected, 0.8, pow2, 2) as t:
+        set_growth(api, t, case)
         o = O.OTable(width)
         sched = O.Sched(expected, 0.8, pow2)
         us = [t.register() for _ in range(users)]
@@ -170,6 +182,7 @@ def run_device_program(api, case):
           f"groups {bool(groups)}, calls {calls}", flush=True)
     L = api.lib
     with capi.Table(api, width, expected, 0.8, pow2, 2) as t:
+        set_growth(api, t, case)
         o = O.OTable(width)
         sched = O.Sched(expected, 0.8, pow2)
         u = t.register()
