@@ -1319,6 +1319,7 @@ struct kache_table
         std::vector<VmSpace::Chunk> chunks;     // in place: physical memory for the store's second half
     };
     std::future<Prepared> prepared;
+    bool prepared_in_place = false;     // the mode the prepared arrays are for
 
     uint64_t next_ov_slots() const { return std::max<uint64_t>(ov_slots_for(std::max<uint64_t>(OV_MIN, 2 * cap)), ov_slots); }
 
@@ -1329,6 +1330,7 @@ struct kache_table
         wait_frees();
         Prepared p;
         p.cap = 2 * cap, p.ov_slots = next_ov_slots(), p.in_place = in_place_next();
+        prepared_in_place = p.in_place;
         p.ov_b = ov_bytes(p.ov_slots);
         if(!p.in_place)
             p.bucket_b = p.cap * bucket_bytes(), p.away_bytes = std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
@@ -1386,7 +1388,10 @@ struct kache_table
         if(2 * cap > MAX_CAP)
             throw hip_failure("growing the table past the most buckets supported");
         wait_frees();
-        if(in_place_next())
+        // Automatically, the mode chosen when the arrays were prepared (the
+        // prepared arrays themselves take free memory the choice looks at).
+        const bool prep = growth_mode == KACHE_AMD_GROW_AUTO && prepared.valid();
+        if(prep ? prepared_in_place : in_place_next())
             resize_in_place();
         else
             resize_out_of_place();
@@ -1423,9 +1428,10 @@ struct kache_table
         {
             if(pre.buckets)
             {
+                // The helper thread took the arrays already: only the budget applies.
                 nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
                 away = static_cast<uint64_t*>(pre.away), away_b = pre.away_bytes;
-                check_budget(nb_bytes + nov_b + away_b, "doubling the table");
+                check_budget(nb_bytes + nov_b + away_b, "doubling the table", 0);
                 zero(nov, nov_b);
             }
             else
@@ -1505,9 +1511,13 @@ struct kache_table
             throw hip_failure("an in-place doubling with too many phases");
         const uint64_t mapped = VmSpace::get().mapped(buckets);
         const uint64_t extra = in_place_extra();
-        check_budget(cap * bb + ov_bytes(nov_slots) + extra, "doubling the table",
-                     (2 * cap * bb > mapped ? 2 * cap * bb - mapped : 0) + ov_bytes(nov_slots) + extra);
         Prepared pre = take_prepared(ncap, nov_slots, true);
+        // Memory the helper thread took already (the second half's chunks,
+        // the new overflow table) is not asked of the device again.
+        uint64_t have = pre.ov ? ov_bytes(nov_slots) : 0;
+        for(const auto& c : pre.chunks)
+            have += c.n;
+        const uint64_t fresh = (2 * cap * bb > mapped ? 2 * cap * bb - mapped : 0) + ov_bytes(nov_slots) + extra;
 
         uint64_t* nov = static_cast<uint64_t*>(pre.ov);
         uint8_t* copy = nullptr;
@@ -1527,6 +1537,7 @@ struct kache_table
         };
         try
         {
+            check_budget(cap * bb + ov_bytes(nov_slots) + extra, "doubling the table", fresh > have ? fresh - have : 0);
             VmSpace::get().grow(buckets, ncap * bb, &pre.chunks);
             VmSpace::release_chunks(pre.chunks);     // any not used
             if(nov)
