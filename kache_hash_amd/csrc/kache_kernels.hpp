@@ -276,221 +276,25 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Resize: doubling into a new, zeroed table of 2 * cap buckets
-// (include/Streaming_Kmer_Hash_Table.hpp:1473-1681), out of place.
-// Phase 1: a key at its primary bucket b moves to 2b or 2b + 1, in its slot
-// order; half a wave per old bucket, so the order is a ballot prefix count.
-// The keys away from their primary bucket are marked in `away`: bit j of
-// word w is slot j % 32 of old bucket 2w + j / 32, written by one lane per
-// wave, with no atomics.
-template <int VB>
-__global__ __launch_bounds__(TILE) void k_split(TableView O, TableView N, uint64_t* away)
-{
-    constexpr uint32_t BB = Geo<VB>::bucket_bytes, IMG = 2 * BB;
-    __shared__ __align__(16) uint8_t images[TILE / 64][2][IMG];
-    const uint32_t lane = threadIdx.x & 63, j = lane & 31;
-    const uint64_t half_mask = (lane < 32) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
-    const uint64_t below = (uint64_t(1) << lane) - 1;
-    const uint64_t pairs = (O.cap + 1) / 2;
-    const uint64_t stride = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-    // Slot j of old bucket 2w + lane / 32: its checksum, word and value, all
-    // read at once (an empty slot's words are 0), a pair ahead of its use.
-    struct Old
-    {
-        uint64_t sk, val;
-        uint32_t c;
-    };
-    const auto read = [&](uint64_t w)
-    {
-        Old o{0, 0, 0};
-        const uint64_t b = 2 * w + (lane >> 5);
-        if(w < pairs && b < O.cap)
-        {
-            uint8_t* ob = bucket_at<VB>(O, b);
-            const Slot<VB> os = slot_ref<VB>(ob, j);
-            o.c = ob[j], o.sk = os.k[0];
-            if constexpr(VB != 0)
-                o.val = *os.v();
-        }
-        return o;
-    };
-    uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6;
-    Old next = read(w);
-    for(; w < pairs; w += stride)
-    {
-        const Old cur = next;
-        next = read(w + stride);
-        const uint64_t b = 2 * w + (lane >> 5);
-        bool lo = false, hi = false, aw = false;
-        const uint64_t sk = cur.sk, val = cur.val;
-        uint8_t c = static_cast<uint8_t>(cur.c);
-        if(c && !is_key(sk))
-            c = 0;      // a tombstone: dropped
-        if(c)
-        {
-            const uint64_t h = minimizer_hash(~sk);
-            if(bucket_of(h, O.cap) == b)
-            {
-                hi = (bucket_of(h, N.cap) == 2 * b + 1);
-                lo = !hi;
-            }
-            else
-                aw = true;
-        }
-        const uint64_t m_lo = __ballot(lo) & half_mask, m_hi = __ballot(hi) & half_mask, m_away = __ballot(aw);
-        // The half's two new buckets (2b, 2b + 1, adjacent in memory) are
-        // assembled in LDS, zeroed, with each moving key at its rank among
-        // those going to the same bucket, then stored whole in 16-byte words;
-        // the new buckets need no zeroing beforehand.
-        uint8_t* img = images[threadIdx.x >> 6][lane >> 5];
-        wave_sync();    // the previous pair's image has been read out
-        for(uint32_t k = j; k < IMG / 16; k += 32)
-            reinterpret_cast<uint4*>(img)[k] = make_uint4(0, 0, 0, 0);
-        wave_sync();
-        if(lo || hi)
-        {
-            const uint32_t r = __popcll((lo ? m_lo : m_hi) & below);
-            uint8_t* nb = img + (lo ? 0 : BB);
-            const Slot<VB> ns = slot_ref<VB>(nb, r);
-            ns.k[0] = sk;
-            if constexpr(VB != 0)
-                *ns.v() = static_cast<typename Geo<VB>::V>(val);
-            nb[r] = c;
-        }
-        wave_sync();
-        if(b < O.cap)
-        {
-            uint4* dst = reinterpret_cast<uint4*>(bucket_at<VB>(N, 2 * b));
-            for(uint32_t k = j; k < IMG / 16; k += 32)
-                dst[k] = reinterpret_cast<const uint4*>(img)[k];
-        }
-        if(lane == 0)
-            away[w] = m_away;
-    }
-}
-
-// Phase 2: the keys marked away from their primary buckets, then the old
-// overflow table's, go in with the insertion policy (:1575-1665).  A wave
-// takes 64 words of marks (128 old buckets), numbers their marked keys by a
-// prefix sum over the lanes, and inserts them 64 at a time, one per lane,
-// with the wave-cooperative claims; then 64 old overflow slots at a time.
-// A claimed slot's value is written by its claimer alone and read by no one
-// before the launch ends, so it takes a plain store (written back with the
-// launch's other stores), not a device-scope one.
-template <int VB>
-__global__ __launch_bounds__(TILE) void k_reinsert(const uint64_t* away, TableView O, TableView N, Counters* cnt)
-{
-    __shared__ uint64_t red[TILE / 64];
-    uint64_t ov_n = 0, failed = 0;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t pairs = (O.cap + 1) / 2, mark_chunks = (pairs + 63) / 64;
-    const uint64_t ov_chunks = (O.ov_mask + 1 + 63) / 64, chunks = mark_chunks + ov_chunks;
-    const uint64_t waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-    for(uint64_t ch = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) >> 6; ch < chunks; ch += waves)
-    {
-        if(ch < mark_chunks)
-        {
-            const uint64_t w = ch * 64 + lane;
-            const uint64_t word = w < pairs ? away[w] : 0;
-            const uint32_t n = __popcll(word);
-            uint32_t incl = n;
-            for(uint32_t o = 1; o < 64; o <<= 1)
-            {
-                const uint32_t t = __shfl_up(incl, o, 64);
-                if(lane >= o)
-                    incl += t;
-            }
-            const uint32_t total = __shfl(incl, 63, 64);
-            for(uint32_t base = 0; base < total; base += 64)
-            {
-                const uint32_t k = base + lane;
-                const bool active = k < total;
-                // The lane holding key k: the first whose inclusive count exceeds k.
-                uint32_t src = 0;
-                for(uint32_t step = 32; step; step >>= 1)
-                    if(__shfl(incl, static_cast<int>(src + step - 1), 64) <= k)
-                        src += step;
-                src = active ? src : 0;
-                uint64_t m = __shfl(static_cast<unsigned long long>(word), static_cast<int>(src), 64);
-                const uint32_t before = __shfl(incl, static_cast<int>(src), 64) - __shfl(n, static_cast<int>(src), 64);
-                uint64_t sk = 0, val = 0;
-                if(active)
-                {
-                    for(uint32_t i = k - before; i; --i)
-                        m &= m - 1;
-                    const uint32_t bit = __builtin_ctzll(m);
-                    const Slot<VB> os = slot_ref<VB>(bucket_at<VB>(O, 2 * (ch * 64 + src) + (bit >> 5)), bit & 31);
-                    sk = os.k[0];
-                    if constexpr(VB != 0)
-                        val = *os.v();
-                }
-                Slot<VB> slot;
-                bool in_ov;
-                const uint64_t key = ~sk;
-                const Res r = insert_new_wave<VB>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
-                if(active)
-                {
-                    if(r == RES_FULL)
-                        failed++;
-                    else
-                    {
-                        ov_n += in_ov;
-                        if constexpr(VB != 0)
-                            *slot.v() = static_cast<typename Geo<VB>::V>(val);
-                    }
-                }
-            }
-        }
-        else
-        {
-            const uint64_t i = (ch - mark_chunks) * 64 + lane;
-            uint64_t sk = 0, val = 0;
-            if(i <= O.ov_mask)
-            {
-                const Slot<VB> os = ov_ref<VB>(O.ov, i);
-                sk = os.k[0];
-                if constexpr(VB != 0)
-                    if(sk)
-                        val = *os.v();
-            }
-            const bool active = sk != 0;
-            Slot<VB> slot;
-            bool in_ov;
-            const uint64_t key = ~sk;
-            const Res r = insert_new_wave<VB>(N, active, key, active ? minimizer_hash(key) : 0, active ? checksum(key) : 0, slot, in_ov);
-            if(active)
-            {
-                if(r == RES_FULL)
-                    failed++;
-                else
-                {
-                    ov_n += in_ov;
-                    if constexpr(VB != 0)
-                        *slot.v() = static_cast<typename Geo<VB>::V>(val);
-                }
-            }
-        }
-    }
-    block_add(ov_n, &cnt->ov_inserted, red);
-    block_add(failed, &cnt->failed, red);
-}
-
-// ---------------------------------------------------------------------------
-// Resize in place (the reference also grows its store in place and splits
-// b -> 2b, 2b + 1 from the top, include/Streaming_Kmer_Hash_Table.hpp:
-// 1473-1681, include/utility.hpp:87-116).  The store's address range is
-// mapped to twice its size first; old bucket b's primaries then move to new
-// buckets 2b and 2b + 1 of the same range.  Phases run from the top: phase
-// [lo, hi) reads old buckets lo .. hi - 1 and writes new buckets 2 lo ..
-// 2 hi - 1, all at or above hi (2 lo >= hi): in the new half, or old buckets
-// that an earlier phase has already read.  A phase never writes its own
-// sources, nor those of a later phase.  The lowest K buckets are copied aside
-// before the first phase and split last, from the copy, so that there are
-// log2(cap / K) + 1 phases rather than log2(cap) + 1.
+// Resize: doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681).  Old
+// bucket b's primaries move to new buckets 2b and 2b + 1, in their slot
+// order; the keys away from their primary bucket are listed and inserted
+// again afterwards, with the old overflow table's.
+//
+// In place (the reference also grows its store in place and splits from the
+// top, include/utility.hpp:87-116): the store's address range is mapped to
+// twice its size first, and the new buckets are written into the same range.
+// Phases run from the top: phase [lo, hi) reads old buckets lo .. hi - 1 and
+// writes new buckets 2 lo .. 2 hi - 1, all at or above hi (2 lo >= hi): in
+// the new half, or old buckets that an earlier phase has already read.  A
+// phase never writes its own sources, nor those of a later phase.  The lowest
+// K buckets are copied aside before the first phase and split last, from the
+// copy, so that there are log2(cap / K) + 1 phases rather than log2(cap) + 1.
+// Out of place: one phase [0, cap) from the old store into a new one.
 //
 // The keys away from their primary bucket cannot wait in their old slots
-// (those are overwritten), so the split lists them with their values in the
-// away list: `regions` regions of `region` entries, region r shared by the
+// (in place those are overwritten), so the split lists them with their values
+// in the away list: `regions` regions of `region` entries, region r shared by the
 // waves whose index is r modulo `regions`.  A wave's step takes entries for
 // its away keys from its region's cursor by one returning atomic, and writes
 // them during the next step, so that the atomic's round trip overlaps a
@@ -514,7 +318,7 @@ struct AwayList
 };
 
 template <int VB>
-__global__ __launch_bounds__(TILE) void k_split_in_place(const uint8_t* src, TableView N, uint64_t old_cap, uint64_t lo, uint64_t hi, AwayList L, uint32_t phase)
+__global__ __launch_bounds__(TILE) void k_split(const uint8_t* src, TableView N, uint64_t old_cap, uint64_t lo, uint64_t hi, AwayList L, uint32_t phase)
 {
     if(__hip_atomic_load(L.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         return;
@@ -675,11 +479,12 @@ __global__ void k_away_regrow(AwayList from, AwayList to, const uint64_t* before
 
 // The keys of the away list and of the old overflow table into the doubled
 // table, with the insertion policy and the reinsert's claims (insert_new_wave).
-// Each key inserted is cleared from its list entry or old overflow slot, so
-// that a run which finds the new overflow table full can be repeated, after
-// that table has grown, for the keys left.
+// With `clear` (in place), each key inserted is cleared from its list entry
+// or old overflow slot, so that a run which finds the new overflow table full
+// can be repeated, after that table has grown, for the keys left; without it
+// (out of place) the old overflow table is only read.
 template <int VB>
-__global__ __launch_bounds__(TILE) void k_reinsert_away(AwayList L, uint64_t* old_ov, uint64_t old_ov_slots, TableView N, Counters* cnt)
+__global__ __launch_bounds__(TILE) void k_reinsert(AwayList L, uint64_t* old_ov, uint64_t old_ov_slots, TableView N, Counters* cnt, bool clear)
 {
     __shared__ uint64_t red[TILE / 64];
     uint64_t ov_n = 0, failed = 0;
@@ -731,7 +536,8 @@ __global__ __launch_bounds__(TILE) void k_reinsert_away(AwayList L, uint64_t* ol
                 ov_n += in_ov;
                 if constexpr(VB != 0)
                     *slot.v() = static_cast<typename Geo<VB>::V>(val);
-                *at = 0;
+                if(clear)
+                    *at = 0;
             }
         }
     }

@@ -1157,7 +1157,7 @@ struct kache_table
 
     uint32_t growth_mode = KACHE_AMD_GROW_AUTO;
     uint64_t away_min = 0;          // entries of an in-place doubling's away list, at least
-    uint64_t in_place = 0;          // doublings done in place
+    uint64_t in_place_doublings = 0;   // doublings done in place
     uint64_t away_regrows = 0;      // times such a doubling's away list grew
     uint64_t last_away = 0;         // keys the last one listed
 
@@ -1182,11 +1182,8 @@ struct kache_table
     }
 
     // HBM an out-of-place doubling needs beside the current arrays: the
-    // doubled store, its overflow table and the split's bitmap.
-    uint64_t out_of_place_bytes() const
-    {
-        return 2 * cap * bucket_bytes() + ov_bytes(next_ov_slots()) + std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
-    }
+    // doubled store, its overflow table and the away list.
+    uint64_t out_of_place_bytes() const { return 2 * cap * bucket_bytes() + ov_bytes(next_ov_slots()) + list_bytes(); }
 
     // Whether the next doubling runs in place.  It can when the store is a
     // range whose reservation holds twice its buckets.  Automatically it
@@ -1222,17 +1219,16 @@ struct kache_table
         return std::max<uint64_t>(uint64_t(AWAY_REGIONS) * 256, static_cast<uint64_t>(static_cast<double>(size) * share));
     }
 
+    // A doubling's away list and its control words.
+    uint64_t list_bytes() const { return away_entries() * (vw ? 16 : 8) + AWAY_CTL_WORDS * 8; }
+
     // Transient arrays of an in-place doubling: the copy of the lowest
     // buckets, the away list and its control words.
-    uint64_t in_place_extra() const
-    {
-        return std::max<uint64_t>(1, cap >> 6) * bucket_bytes() + away_entries() * (vw ? 16 : 8) + AWAY_CTL_WORDS * 8;
-    }
+    uint64_t in_place_extra() const { return std::max<uint64_t>(1, cap >> 6) * bucket_bytes() + list_bytes(); }
 
     // HBM the next doubling needs beside the current arrays.  In place: the
     // store's second half, the new overflow table and the transient arrays.
-    // Out of place: the doubled store, its overflow table and the split's
-    // bitmap.
+    // Out of place: the doubled store, its overflow table and the away list.
     uint64_t next_doubling_bytes() const
     {
         if(in_place_next())
@@ -1311,11 +1307,10 @@ struct kache_table
     // otherwise stall the launches around the doubling.
     struct Prepared
     {
-        uint64_t cap = 0, ov_slots = 0, bucket_b = 0, ov_b = 0, away_bytes = 0;
-        bool in_place = false;      // only the overflow table and the store's new chunks are prepared
+        uint64_t cap = 0, ov_slots = 0, bucket_b = 0, ov_b = 0;
+        bool in_place = false;      // in place: the overflow table and the store's new chunks; else the new store too
         void* buckets = nullptr;
         void* ov = nullptr;
-        void* away = nullptr;
         std::vector<VmSpace::Chunk> chunks;     // in place: physical memory for the store's second half
     };
     std::future<Prepared> prepared;
@@ -1333,7 +1328,7 @@ struct kache_table
         prepared_in_place = p.in_place;
         p.ov_b = ov_bytes(p.ov_slots);
         if(!p.in_place)
-            p.bucket_b = p.cap * bucket_bytes(), p.away_bytes = std::max<uint64_t>((cap + 1) / 2 * 8, 4096);
+            p.bucket_b = p.cap * bucket_bytes();
         if(hbm_budget != 0 && device_bytes() + next_doubling_bytes() > hbm_budget)
             return;     // the doubling would be refused: nothing to prepare
         const bool store_range = !p.in_place && growable(p.bucket_b);
@@ -1349,10 +1344,7 @@ struct kache_table
                 if(p.in_place && mapped < target)
                     p.chunks = VmSpace::make_chunks(dev, mapped, target);
                 if(!p.in_place)
-                {
                     p.buckets = store_range ? ArrayCache::get().take_range(dev, p.bucket_b, reserve) : ArrayCache::get().take(dev, p.bucket_b);
-                    p.away = ArrayCache::get().take(dev, p.away_bytes);
-                }
             }
             catch(...)
             {
@@ -1366,7 +1358,6 @@ struct kache_table
         VmSpace::release_chunks(p.chunks);
         ArrayCache::get().give(device, p.buckets, p.bucket_b);
         ArrayCache::get().give(device, p.ov, p.ov_b);
-        ArrayCache::get().give(device, p.away, p.away_bytes);
     }
 
     // The prepared arrays, if they are those of the doubling about to run.
@@ -1375,7 +1366,7 @@ struct kache_table
         if(!prepared.valid())
             return {};
         Prepared p = prepared.get();
-        if(p.cap != ncap || p.ov_slots != nov_slots || p.in_place != in_place_now || !p.ov || (!p.in_place && (!p.buckets || !p.away)))
+        if(p.cap != ncap || p.ov_slots != nov_slots || p.in_place != in_place_now || !p.ov || (!p.in_place && !p.buckets))
         {
             free_prepared(p);
             return {};
@@ -1391,134 +1382,63 @@ struct kache_table
         // Automatically, the mode chosen when the arrays were prepared (the
         // prepared arrays themselves take free memory the choice looks at).
         const bool prep = growth_mode == KACHE_AMD_GROW_AUTO && prepared.valid();
-        if(prep ? prepared_in_place : in_place_next())
-            resize_in_place();
-        else
-            resize_out_of_place();
+        resize_split(prep ? prepared_in_place : in_place_next());
         resizes++;
         resize_th = static_cast<uint64_t>(static_cast<double>(cap * B) * lf);
         step = step_for(resize_th);
     }
 
-    // Doubling into a new store: k_split moves the primaries, the keys away
-    // from their primary bucket are marked in a bitmap, k_reinsert inserts
-    // them and the old overflow table's keys.  The old arrays are only read,
-    // so a failure leaves the table as it was.
-    void resize_out_of_place()
-    {
-        const uint64_t ncap = 2 * cap;
-        // :1387-1392: the new overflow table takes the old one's keys' room, at least.
-        const uint64_t nov_slots = next_ov_slots();
-        const uint64_t pairs = (cap + 1) / 2;
-        const Prepared pre = take_prepared(ncap, nov_slots, false);
-        const uint64_t nb_bytes = ncap * bucket_bytes(), nov_b = ov_bytes(nov_slots);
-        uint8_t* nb = nullptr;      // written whole by the split
-        uint64_t* nov = nullptr;
-        // One bit per old slot: the keys away from their primary bucket.
-        uint64_t* away = nullptr;
-        uint64_t away_b = std::max<uint64_t>(pairs * 8, 4096);
-        const auto drop_new = [&]
-        {
-            (void)hipStreamSynchronize(stream);
-            ArrayCache::get().give(device, nb, nb_bytes), nb = nullptr;
-            ArrayCache::get().give(device, nov, nov_b), nov = nullptr;
-            ArrayCache::get().give(device, away, away_b), away = nullptr;
-        };
-        try
-        {
-            if(pre.buckets)
-            {
-                // The helper thread took the arrays already: only the budget applies.
-                nb = static_cast<uint8_t*>(pre.buckets), nov = static_cast<uint64_t*>(pre.ov);
-                away = static_cast<uint64_t*>(pre.away), away_b = pre.away_bytes;
-                check_budget(nb_bytes + nov_b + away_b, "doubling the table", 0);
-                zero(nov, nov_b);
-            }
-            else
-            {
-                check_budget(nb_bytes + nov_b + away_b, "doubling the table");
-                nb = take_buckets(ncap);
-                nov = alloc_ov(nov_slots);
-                away = static_cast<uint64_t*>(ArrayCache::get().take(device, away_b));
-            }
-        }
-        catch(...)
-        {
-            drop_new();
-            throw;
-        }
-
-        Counters c;
-        try
-        {
-            const TableView O = view(), N = view_of(nb, ncap, nov, nov_slots);
-            const uint64_t split_blocks = (pairs * 64 + TILE - 1) / TILE;
-            const uint64_t re_blocks = (pairs + ov_slots + TILE - 1) / TILE;
-            timed("split", [&]{
-                by_layout([&](auto L){
-                    const auto kern = k_split<decltype(L)::value>;
-                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, split_blocks)), dim3(TILE), 0, stream, O, N, away);
-                });
-            });
-            timed("reinsert", [&]{
-                by_layout([&](auto L){
-                    const auto kern = k_reinsert<decltype(L)::value>;
-                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, re_blocks)), dim3(TILE), 0, stream, away, O, N, d_cnt);
-                });
-            });
-            c = read_counters();
-            if(c.failed)
-                throw hip_failure("resize found no room in the overflow table");
-        }
-        catch(...)
-        {
-            // The split and reinsert only read the old arrays: the table is
-            // unchanged, and the new arrays go.
-            drop_new();
-            throw;
-        }
-
-        free_later({{buckets, cap * bucket_bytes()}, {ov, ov_bytes(ov_slots)}, {away, away_b}});
-        buckets = nb, cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = c.ov_inserted;
-    }
-
-    // Doubling in place (k_split_in_place): the store's range is mapped to
-    // twice its buckets, the lowest K = cap / 64 buckets are copied aside,
-    // the phases split the store from the top and the copy last, listing the
-    // keys away from their primary bucket; k_reinsert_away inserts those and
-    // the old overflow table's keys.  Peak memory: the doubled store, the
-    // two overflow tables, the copy (1/64 of the store) and the away list
-    // (16 bytes per listed key), instead of the old and new stores side by
-    // side.  Once the first phase has run the old buckets are being
-    // overwritten: a device error from there on leaves the table unusable.
-    // A full away list region, or a new overflow table with no room left for
-    // the reinsert, is handled by growing the array and running again.
-    void resize_in_place()
+    // Doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681) by
+    // k_split and k_reinsert.
+    // In place: the store's range is mapped to twice its buckets, the lowest
+    // K = cap / 64 buckets are copied aside, the phases split the store from
+    // the top and the copy last.  Peak memory: the doubled store, the two
+    // overflow tables, the copy and the away list.  Once the first phase has
+    // run the old buckets are being overwritten: a device error from there on
+    // leaves the table unusable.
+    // Out of place: one phase [0, cap) from the old store into a new one of
+    // twice its size; the old arrays are only read, so any failure leaves the
+    // table as it was and the new arrays go.
+    // Either way the split lists the keys away from their primary bucket, with
+    // their values; k_reinsert inserts those and the old overflow
+    // table's keys.  A full away list region, or a new overflow table with no
+    // room left for the reinsert, is handled by growing the array and running
+    // that step again.
+    void resize_split(bool in_place)
     {
         const uint32_t bb = bucket_bytes();
         const uint64_t ncap = 2 * cap;
         uint64_t nov_slots = next_ov_slots();
-        const uint64_t K = std::max<uint64_t>(1, cap >> 6);
+        const uint64_t K = in_place ? std::max<uint64_t>(1, cap >> 6) : 0;
         std::vector<std::pair<uint64_t, uint64_t>> phases;
-        for(uint64_t hi = cap; hi > K;)
+        if(in_place)
         {
-            const uint64_t lo = std::max(K, (hi + 1) / 2);
-            phases.push_back({lo, hi});
-            hi = lo;
+            for(uint64_t hi = cap; hi > K;)
+            {
+                const uint64_t lo = std::max(K, (hi + 1) / 2);
+                phases.push_back({lo, hi});
+                hi = lo;
+            }
+            phases.push_back({0, K});
         }
-        phases.push_back({0, K});
+        else
+            phases.push_back({0, cap});
         if(phases.size() > MAX_PHASES)
             throw hip_failure("an in-place doubling with too many phases");
-        const uint64_t mapped = VmSpace::get().mapped(buckets);
-        const uint64_t extra = in_place_extra();
-        Prepared pre = take_prepared(ncap, nov_slots, true);
-        // Memory the helper thread took already (the second half's chunks,
-        // the new overflow table) is not asked of the device again.
-        uint64_t have = pre.ov ? ov_bytes(nov_slots) : 0;
+        const uint64_t mapped = in_place ? VmSpace::get().mapped(buckets) : 0;
+        const uint64_t extra = in_place ? in_place_extra() : list_bytes();
+        Prepared pre = take_prepared(ncap, nov_slots, in_place);
+        // Memory the helper thread took already (the second half's chunks or
+        // the new store, the new overflow table) is not asked of the device
+        // again.
+        uint64_t have = (pre.ov ? ov_bytes(nov_slots) : 0) + (pre.buckets ? ncap * bb : 0);
         for(const auto& c : pre.chunks)
             have += c.n;
-        const uint64_t fresh = (2 * cap * bb > mapped ? 2 * cap * bb - mapped : 0) + ov_bytes(nov_slots) + extra;
+        const uint64_t store_more = in_place ? cap * bb : ncap * bb;
+        const uint64_t store_fresh = in_place ? (2 * cap * bb > mapped ? 2 * cap * bb - mapped : 0) : ncap * bb;
+        const uint64_t fresh = store_fresh + ov_bytes(nov_slots) + extra;
 
+        uint8_t* dst = in_place ? buckets : static_cast<uint8_t*>(pre.buckets);   // the doubled store
         uint64_t* nov = static_cast<uint64_t*>(pre.ov);
         uint8_t* copy = nullptr;
         uint64_t* ctl = nullptr;
@@ -1529,6 +1449,8 @@ struct kache_table
         {
             (void)hipStreamSynchronize(stream);
             VmSpace::release_chunks(pre.chunks);
+            if(!in_place)
+                ArrayCache::get().give(device, dst, ncap * bb), dst = nullptr;
             ArrayCache::get().give(device, nov, ov_bytes(nov_slots)), nov = nullptr;
             ArrayCache::get().give(device, copy, K * bb), copy = nullptr;
             ArrayCache::get().give(device, ctl, AWAY_CTL_WORDS * 8), ctl = nullptr;
@@ -1537,14 +1459,20 @@ struct kache_table
         };
         try
         {
-            check_budget(cap * bb + ov_bytes(nov_slots) + extra, "doubling the table", fresh > have ? fresh - have : 0);
-            VmSpace::get().grow(buckets, ncap * bb, &pre.chunks);
-            VmSpace::release_chunks(pre.chunks);     // any not used
+            check_budget(store_more + ov_bytes(nov_slots) + extra, "doubling the table", fresh > have ? fresh - have : 0);
+            if(in_place)
+            {
+                VmSpace::get().grow(buckets, ncap * bb, &pre.chunks);
+                VmSpace::release_chunks(pre.chunks);     // any not used
+            }
+            else if(!dst)
+                dst = take_buckets(ncap);
             if(nov)
                 zero(nov, ov_bytes(nov_slots));
             else
                 nov = alloc_ov(nov_slots);
-            copy = static_cast<uint8_t*>(ArrayCache::get().take(device, K * bb));
+            if(in_place)
+                copy = static_cast<uint8_t*>(ArrayCache::get().take(device, K * bb));
             ctl = static_cast<uint64_t*>(ArrayCache::get().take(device, AWAY_CTL_WORDS * 8));
             lkeys = static_cast<uint64_t*>(ArrayCache::get().take(device, entries * 8));
             if(vw)
@@ -1552,114 +1480,140 @@ struct kache_table
         }
         catch(...)
         {
-            // Nothing of the table has changed (its range may stay mapped further).
+            // Nothing of the table has changed (in place, its range may stay
+            // mapped further).
             give_new();
             throw;
         }
 
-        // Every phase runs on the first (largest) phase's grid, so that the
-        // waves of each away list region are the same in every phase.
-        uint32_t grid = 0;
-        by_layout([&](auto L){
-            const auto kern = k_split_in_place<decltype(L)::value>;
-            const uint64_t pairs = (phases[0].second - phases[0].first + 1) / 2;
-            grid = grid_for(kern, (pairs * 64 + TILE - 1) / TILE);
-        });
-        const uint32_t regions = static_cast<uint32_t>(std::min<uint64_t>(AWAY_REGIONS, uint64_t(grid) * (TILE / 64)));
-        AwayList L{lkeys, lvals, entries / regions, ctl, reinterpret_cast<uint32_t*>(ctl + AWAY_REGIONS * AWAY_CURSOR_STRIDE), regions};
-        uint64_t* snaps = ctl + AWAY_REGIONS * AWAY_CURSOR_STRIDE + 8;
-        const TableView N = view_of(buckets, ncap, nov, nov_slots);
-        const auto run_phases = [&](size_t from)
+        try
         {
-            for(size_t k = from; k < phases.size(); ++k)
+            // Every phase runs on the first (largest) phase's grid, so that
+            // the waves of each away list region are the same in every phase.
+            uint32_t grid = 0;
+            by_layout([&](auto V){
+                const auto kern = k_split<decltype(V)::value>;
+                const uint64_t pairs = (phases[0].second - phases[0].first + 1) / 2;
+                grid = grid_for(kern, (pairs * 64 + TILE - 1) / TILE);
+            });
+            const uint32_t regions = static_cast<uint32_t>(std::min<uint64_t>(AWAY_REGIONS, uint64_t(grid) * (TILE / 64)));
+            AwayList L{lkeys, lvals, entries / regions, ctl, reinterpret_cast<uint32_t*>(ctl + AWAY_REGIONS * AWAY_CURSOR_STRIDE), regions};
+            uint64_t* snaps = ctl + AWAY_REGIONS * AWAY_CURSOR_STRIDE + 8;
+            const TableView N = view_of(dst, ncap, nov, nov_slots);
+            const auto run_phases = [&](size_t from)
             {
-                hipLaunchKernelGGL(k_away_snap, dim3(1), dim3(256), 0, stream, static_cast<const uint64_t*>(L.cursor), regions, snaps + k * AWAY_REGIONS);
-                const uint8_t* src = k + 1 == phases.size() ? copy : buckets;
-                timed("split", [&]{
+                for(size_t k = from; k < phases.size(); ++k)
+                {
+                    hipLaunchKernelGGL(k_away_snap, dim3(1), dim3(256), 0, stream, static_cast<const uint64_t*>(L.cursor), regions,
+                                       snaps + k * AWAY_REGIONS);
+                    const uint8_t* src = (in_place && k + 1 == phases.size()) ? copy : buckets;
+                    timed("split", [&]{
+                        by_layout([&](auto V){
+                            hipLaunchKernelGGL(k_split<decltype(V)::value>, dim3(grid), dim3(TILE), 0, stream, src, N, cap,
+                                               phases[k].first, phases[k].second, L, static_cast<uint32_t>(k));
+                        });
+                    });
+                }
+            };
+            zero(ctl, AWAY_CTL_WORDS * 8);
+            if(in_place)
+                HIP(hipMemcpyAsync(copy, buckets, K * bb, hipMemcpyDeviceToDevice, stream));
+            run_phases(0);
+            constexpr uint64_t CUR_WORDS = AWAY_REGIONS * AWAY_CURSOR_STRIDE;
+            h_away.reserve((CUR_WORDS + 1) * 8);
+            uint64_t listed = 0;
+            for(;;)
+            {
+                // The cursors and the fail word, in one copy.
+                HIP(hipMemcpyAsync(h_away.p, ctl, (CUR_WORDS + 1) * 8, hipMemcpyDeviceToHost, stream));
+                sync();
+                const uint64_t* hc = h_away.as<uint64_t>();
+                const uint64_t fail = hc[CUR_WORDS] & 0xFFFFFFFFull;
+                if(fail == 0)
+                {
+                    for(uint32_t r = 0; r < regions; ++r)
+                        listed += hc[uint64_t(r) * AWAY_CURSOR_STRIDE];
+                    break;
+                }
+                // Phase fail - 1 found its region full: grow the list, keep
+                // each region's entries from before that phase, and run it
+                // again (its sources are untouched).
+                const size_t k = fail - 1;
+                const uint64_t region2 = 2 * L.region, entries2 = region2 * regions;
+                AwayList L2 = L;
+                L2.keys = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
+                L2.vals = vw ? static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8)) : nullptr;
+                L2.region = region2;
+                hipLaunchKernelGGL(k_away_regrow, dim3(std::min<uint64_t>(1024, (L.region + 255) / 256 + 1)), dim3(256), 0, stream, L, L2,
+                                   static_cast<const uint64_t*>(snaps + k * AWAY_REGIONS));
+                HIP(hipGetLastError());
+                sync();
+                ArrayCache::get().give(device, lkeys, entries * 8);
+                ArrayCache::get().give(device, lvals, entries * 8);
+                lkeys = L2.keys, lvals = L2.vals, entries = entries2;
+                L = L2;
+                away_regrows++;
+                run_phases(k);
+            }
+
+            // The listed keys and the old overflow table's, into the doubled table.
+            uint64_t ov_n = 0;
+            for(int attempt = 0;; ++attempt)
+            {
+                const TableView NV = view_of(dst, ncap, nov, nov_slots);
+                const uint64_t chunks = uint64_t(regions) * ((L.region + 63) / 64) + (ov_slots + 63) / 64;
+                timed("reinsert", [&]{
                     by_layout([&](auto V){
-                        hipLaunchKernelGGL(k_split_in_place<decltype(V)::value>, dim3(grid), dim3(TILE), 0, stream, src, N, cap,
-                                           phases[k].first, phases[k].second, L, static_cast<uint32_t>(k));
+                        const auto kern = k_reinsert<decltype(V)::value>;
+                        hipLaunchKernelGGL(kern, dim3(grid_for(kern, (chunks * 64 + TILE - 1) / TILE)), dim3(TILE), 0, stream, L, ov, ov_slots,
+                                           NV, d_cnt, in_place);
                     });
                 });
-            }
-        };
-        zero(ctl, AWAY_CTL_WORDS * 8);
-        HIP(hipMemcpyAsync(copy, buckets, K * bb, hipMemcpyDeviceToDevice, stream));
-        run_phases(0);
-        constexpr uint64_t CUR_WORDS = AWAY_REGIONS * AWAY_CURSOR_STRIDE;
-        h_away.reserve((CUR_WORDS + 1) * 8);
-        uint64_t listed = 0;
-        for(;;)
-        {
-            // The cursors and the fail word, in one copy.
-            HIP(hipMemcpyAsync(h_away.p, ctl, (CUR_WORDS + 1) * 8, hipMemcpyDeviceToHost, stream));
-            sync();
-            const uint64_t* hc = h_away.as<uint64_t>();
-            const uint64_t fail = hc[CUR_WORDS] & 0xFFFFFFFFull;
-            if(fail == 0)
-            {
-                for(uint32_t r = 0; r < regions; ++r)
-                    listed += hc[uint64_t(r) * AWAY_CURSOR_STRIDE];
-                break;
-            }
-            // Phase fail - 1 found its region full: grow the list, keep each
-            // region's entries from before that phase, and run it again.
-            const size_t k = fail - 1;
-            const uint64_t region2 = 2 * L.region, entries2 = region2 * regions;
-            AwayList L2 = L;
-            L2.keys = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
-            L2.vals = vw ? static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8)) : nullptr;
-            L2.region = region2;
-            hipLaunchKernelGGL(k_away_regrow, dim3(std::min<uint64_t>(1024, (L.region + 255) / 256 + 1)), dim3(256), 0, stream, L, L2,
-                               static_cast<const uint64_t*>(snaps + k * AWAY_REGIONS));
-            HIP(hipGetLastError());
-            sync();
-            ArrayCache::get().give(device, lkeys, entries * 8);
-            ArrayCache::get().give(device, lvals, entries * 8);
-            lkeys = L2.keys, lvals = L2.vals, entries = entries2;
-            L = L2;
-            away_regrows++;
-            run_phases(k);
-        }
-
-        // The listed keys and the old overflow table's, into the doubled table.
-        uint64_t ov_n = 0;
-        for(int attempt = 0;; ++attempt)
-        {
-            const TableView NV = view_of(buckets, ncap, nov, nov_slots);
-            const uint64_t chunks = uint64_t(regions) * ((L.region + 63) / 64) + (ov_slots + 63) / 64;
-            timed("reinsert", [&]{
+                const Counters c = read_counters();
+                ov_n += c.ov_inserted;
+                if(!c.failed)
+                    break;
+                // Out of place the table is left as it was; in place the keys
+                // left are tried again.
+                if(!in_place || attempt == 8)
+                    throw hip_failure("a doubling found no room in the overflow table");
+                // The keys left stay listed: a larger new overflow table, then again.
+                uint64_t* nov2 = alloc_ov(2 * nov_slots);
+                const TableView N2 = view_of(dst, ncap, nov2, 2 * nov_slots);
                 by_layout([&](auto V){
-                    const auto kern = k_reinsert_away<decltype(V)::value>;
-                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, (chunks * 64 + TILE - 1) / TILE)), dim3(TILE), 0, stream, L, ov, ov_slots, NV, d_cnt);
+                    const auto kern = k_ov_rehash<decltype(V)::value>;
+                    hipLaunchKernelGGL(kern, dim3(grid_for(kern, (nov_slots + TILE - 1) / TILE)), dim3(TILE), 0, stream, nov, nov_slots, N2, d_cnt);
                 });
-            });
-            const Counters c = read_counters();
-            ov_n += c.ov_inserted;
-            if(!c.failed)
-                break;
-            if(attempt == 8)
-                throw hip_failure("an in-place doubling found no room in the overflow table");
-            // The keys left stay listed: a larger new overflow table, then again.
-            uint64_t* nov2 = alloc_ov(2 * nov_slots);
-            const TableView N2 = view_of(buckets, ncap, nov2, 2 * nov_slots);
-            by_layout([&](auto V){
-                const auto kern = k_ov_rehash<decltype(V)::value>;
-                hipLaunchKernelGGL(kern, dim3(grid_for(kern, (nov_slots + TILE - 1) / TILE)), dim3(TILE), 0, stream, nov, nov_slots, N2, d_cnt);
-            });
-            HIP(hipGetLastError());
-            if(read_counters().failed)
-                throw hip_failure("an in-place doubling found no room in the overflow table");
-            ArrayCache::get().give(device, nov, ov_bytes(nov_slots));
-            nov = nov2, nov_slots *= 2;
-        }
+                HIP(hipGetLastError());
+                if(read_counters().failed)
+                {
+                    ArrayCache::get().give(device, nov2, ov_bytes(2 * nov_slots));
+                    throw hip_failure("a doubling found no room in the overflow table");
+                }
+                ArrayCache::get().give(device, nov, ov_bytes(nov_slots));
+                nov = nov2, nov_slots *= 2;
+            }
 
-        free_later({{ov, ov_bytes(ov_slots)}, {copy, K * bb}, {ctl, AWAY_CTL_WORDS * 8}, {lkeys, entries * 8}, {lvals, entries * 8}});
-        last_away = listed;
-        if(size > 0)
-            away_share = static_cast<double>(listed) / static_cast<double>(size);
-        cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = ov_n;
-        in_place++;
+            std::vector<std::pair<void*, uint64_t>> old = {{ov, ov_bytes(ov_slots)}, {copy, K * bb}, {ctl, AWAY_CTL_WORDS * 8},
+                                                           {lkeys, entries * 8}, {lvals, entries * 8}};
+            if(!in_place)
+                old.push_back({buckets, cap * bb});
+            free_later(old);
+            last_away = listed;
+            if(size > 0)
+                away_share = static_cast<double>(listed) / static_cast<double>(size);
+            buckets = dst, cap = ncap, ov = nov, ov_slots = nov_slots, ov_count = ov_n;
+            if(in_place)
+                in_place_doublings++;
+        }
+        catch(...)
+        {
+            // Out of place the old arrays were only read: the table is as it
+            // was, and the new arrays go.
+            if(!in_place)
+                give_new();
+            throw;
+        }
     }
 
     // The growth schedule of one insert call by one user: the keys it inserts
@@ -3899,7 +3853,7 @@ int kache_amd_get_info(const kache_table* t, kache_amd_info* out) noexcept
     if(out->store_mapped_bytes == 0)
         out->store_mapped_bytes = t->cap * t->bucket_bytes();
     out->store_reserved_bytes = VmSpace::get().reserved(t->buckets);
-    out->in_place_doublings = t->in_place;
+    out->in_place_doublings = t->in_place_doublings;
     out->away_list_regrows = t->away_regrows;
     out->last_away_keys = t->last_away;
     out->growth_mode = t->growth_mode;
