@@ -463,7 +463,7 @@ void dev_free(void* p)
         (void)hipFree(p);
 }
 
-// Large device arrays (bucket stores, overflow tables, doubling bitmaps) are
+// Large device arrays (bucket stores, overflow tables, doubling away lists) are
 // kept when a table frees them and handed out again for an array of the same
 // size on the same device, so a new table of the same shape, and the same
 // growth steps, reuse them.  The driver wipes device memory it gets back
