@@ -162,7 +162,9 @@ int kache_amd_set_blocks_per_cu(kache_table* t, uint32_t blocks) KACHE_NOEXCEPT;
  * allocates, with KACHE_ENOMEM: the call's launches so far stay applied (as
  * a call that fails part-way in the reference), the table keeps its capacity
  * and every result stays exact, and the next flush of the growth schedule
- * tries the doubling again.  A store taken from memory kept by earlier tables
+ * tries the doubling again (the one exception: an in-place doubling's away
+ * list that fills up grows beyond the budget, see kache_amd_set_growth).  A
+ * store taken from memory kept by earlier tables
  * may come mapped beyond its buckets (room an in-place doubling grows into);
  * setting a budget returns that room to the device, to within one chunk of at
  * most 1 GiB. */
@@ -185,8 +187,17 @@ int kache_amd_set_hbm_budget(kache_table* t, uint64_t bytes) KACHE_NOEXCEPT;
  *                                grow in place moves out of place once, into one that can
  *   KACHE_AMD_GROW_OUT_OF_PLACE  always out of place
  * away_entries: the first size of an in-place doubling's list of away keys
- * (0: the default, an eighth of the keys and at least 65,536; at least 256
- * otherwise); a list that fills up is grown and the doubling goes on. */
+ * (0: the default, 1.25 times the share of keys the table's last in-place
+ * doubling listed plus 1% of the keys, 39% at first, and at least 65,536; at
+ * least 256 otherwise); a list that fills up is grown and the doubling goes
+ * on, beyond the table's budget if need be (the doubling cannot stop there).
+ * An in-place doubling that fails once its split has begun (a device error,
+ * or no device memory for a list that must grow) leaves the keys partly split:
+ * its call fails, and every later call that reads or changes the table's
+ * keys (inserts, updates, queries, kache_for_each, kache_get_stats,
+ * kache_size, kache_amd_save ...) returns KACHE_EINTERNAL with
+ * kache_amd_last_error saying so; kache_amd_get_info and kache_free still
+ * work.  Out of place a failed doubling leaves the table as it was. */
 enum
 {
     KACHE_AMD_GROW_AUTO = 0,

@@ -1148,8 +1148,17 @@ struct kache_table
     // --- Device-memory budget ---
 
     // Most bytes of HBM the table may hold at once, its transient arrays
-    // during growth included (0: no limit but the device's own).
+    // during growth included (0: no limit but the device's own).  One
+    // exception: an away list that must grow part way through an in-place
+    // doubling grows beyond it, since that doubling cannot stop there.
     uint64_t hbm_budget = 0;
+
+    // Set when an in-place doubling failed after its first phase had
+    // overwritten old buckets: the table's keys are then partly split, and
+    // every later call that reads or changes them is refused with `damage`
+    // (written before the flag is set).
+    std::atomic<bool> damaged{false};
+    std::string damage;
 
     uint64_t device_bytes() const { return cap * bucket_bytes() + ov_bytes(ov_slots); }
 
@@ -1540,10 +1549,27 @@ struct kache_table
                 // again (its sources are untouched).
                 const size_t k = fail - 1;
                 const uint64_t region2 = 2 * L.region, entries2 = region2 * regions;
+                if(std::getenv("KACHE_AMD_TEST_FAIL_AWAY_REGROW") != nullptr)
+                {
+                    last_error = "growing a doubling's away list: refused (KACHE_AMD_TEST_FAIL_AWAY_REGROW is set)";
+                    throw std::bad_alloc();
+                }
                 AwayList L2 = L;
                 L2.keys = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
-                L2.vals = vw ? static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8)) : nullptr;
+                L2.vals = nullptr;
                 L2.region = region2;
+                if(vw)
+                {
+                    try
+                    {
+                        L2.vals = static_cast<uint64_t*>(ArrayCache::get().take(device, entries2 * 8));
+                    }
+                    catch(...)
+                    {
+                        ArrayCache::get().give(device, L2.keys, entries2 * 8);
+                        throw;
+                    }
+                }
                 hipLaunchKernelGGL(k_away_regrow, dim3(std::min<uint64_t>(1024, (L.region + 255) / 256 + 1)), dim3(256), 0, stream, L, L2,
                                    static_cast<const uint64_t*>(snaps + k * AWAY_REGIONS));
                 HIP(hipGetLastError());
@@ -1609,9 +1635,30 @@ struct kache_table
         catch(...)
         {
             // Out of place the old arrays were only read: the table is as it
-            // was, and the new arrays go.
-            if(!in_place)
-                give_new();
+            // was, and the new arrays go.  In place the first phase has
+            // overwritten old buckets: the table is refused from now on, and
+            // the transient arrays go (the store stays, for kache_free).
+            if(in_place)
+            {
+                std::string why = last_error;
+                try
+                {
+                    throw;
+                }
+                catch(const std::bad_alloc&)
+                {
+                }
+                catch(const std::exception& e)
+                {
+                    why = e.what();
+                }
+                catch(...)
+                {
+                }
+                damage = "the table is unusable: an in-place doubling failed part way (" + (why.empty() ? std::string("out of memory") : why) + ")";
+                damaged.store(true, std::memory_order_release);
+            }
+            give_new();
             throw;
         }
     }
@@ -3165,6 +3212,19 @@ bool fits(const kache_table* t, uint64_t x)
 // capi/kache_capi.cpp:57-60.
 bool registered(const kache_table* t, uint32_t user) { return user < t->user_c.load(std::memory_order_acquire); }
 
+// guard() for a call that reads or changes the keys of table `t`: a table
+// that a failed in-place doubling left partly split refuses it.
+template <typename F>
+int guard(const kache_table* t, const F& f) noexcept
+{
+    if(t->damaged.load(std::memory_order_acquire))
+    {
+        last_error = t->damage;
+        return KACHE_EINTERNAL;
+    }
+    return guard(f);
+}
+
 bool device_text_ok(const char* text, uint64_t bytes)
 {
     return bytes == 0 || (text != nullptr && reinterpret_cast<uintptr_t>(text) % 16 == 0);
@@ -3236,7 +3296,7 @@ int kache_insert(kache_table* t, uint32_t user, const char* const* seqs, const s
 {
     if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) || (t->value_bytes > 0 && (op > KACHE_MAX || !fits(t, x))))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = t->insert_host(user, seqs, lens, n, t->map ? op : 0, false, t->map ? x : 0, 0);
         if(inserted != nullptr)
@@ -3250,7 +3310,7 @@ int kache_insert_oriented(kache_table* t, uint32_t user, const char* const* seqs
     const bool op_ok = (op == KACHE_ASSIGN || op == KACHE_ADD || op == KACHE_OR || op == KACHE_MAX);
     if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) || !op_ok || !fits(t, x_fwd) || !fits(t, x_rev))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = t->insert_host(user, seqs, lens, n, op, true, x_fwd, x_rev);
         if(inserted != nullptr)
@@ -3263,7 +3323,7 @@ int kache_update(kache_table* t, const char* const* seqs, const size_t* lens, si
 {
     if(t == nullptr || !valid(seqs, lens, n) || op > KACHE_MAX || !fits(t, x))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = t->update_host(seqs, lens, n, op, x);
         if(present != nullptr)
@@ -3276,7 +3336,7 @@ int kache_count(const kache_table* t, const char* const* seqs, const size_t* len
 {
     if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && counts == nullptr))
         return KACHE_EINVAL;
-    return guard([&](){ t->per_sequence<Q_COUNT>(seqs, lens, n, counts, 0, 0); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->per_sequence<Q_COUNT>(seqs, lens, n, counts, 0, 0); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_count_masked(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint64_t mask_fwd, uint64_t mask_rev, uint64_t* counts) noexcept
@@ -3285,21 +3345,21 @@ int kache_count_masked(const kache_table* t, const char* const* seqs, const size
         return KACHE_EINVAL;
     // Masks act on values of the table's width (capi/Kmer_Table.hpp:146).
     const uint64_t m = t->vmax();
-    return guard([&](){ t->per_sequence<Q_MASKED>(seqs, lens, n, counts, mask_fwd & m, mask_rev & m); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->per_sequence<Q_MASKED>(seqs, lens, n, counts, mask_fwd & m, mask_rev & m); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_sum(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint64_t* sums) noexcept
 {
     if(t == nullptr || t->value_bytes == 0 || !valid(seqs, lens, n) || (n > 0 && sums == nullptr))
         return KACHE_EINVAL;
-    return guard([&](){ t->per_sequence<Q_SUM>(seqs, lens, n, sums, 0, 0); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->per_sequence<Q_SUM>(seqs, lens, n, sums, 0, 0); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_lookup(const kache_table* t, const char* const* seqs, const size_t* lens, size_t n, uint8_t* found, void* values, uint8_t* fwd) noexcept
 {
     if(t == nullptr || !valid(seqs, lens, n) || (n > 0 && found == nullptr) || (t->value_bytes == 0 && values != nullptr))
         return KACHE_EINVAL;
-    return guard([&](){ t->lookup(seqs, lens, n, found, values, fwd); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->lookup(seqs, lens, n, found, values, fwd); return static_cast<int>(KACHE_OK); });
 }
 
 namespace
@@ -3323,7 +3383,7 @@ int kache_amd_count_seqs_device(const kache_table* t, const char* text, uint64_t
 {
     if(t == nullptr || !offsets_ok(text, text_bytes, offsets, n) || (n > 0 && counts == nullptr))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         t->per_sequence_device<Q_COUNT>(reinterpret_cast<const uint8_t*>(text), offsets, n, counts, 0, 0);
         return static_cast<int>(KACHE_OK);
@@ -3342,7 +3402,7 @@ int kache_amd_count_seqs_device_static(const kache_table* t, const char* text, u
                      std::to_string(STATIC_MAX_SEQS) + " sequences and " + std::to_string(STATIC_MAX_TEXT) + " bytes)";
         return KACHE_EINVAL;
     }
-    return guard([&]()
+    return guard(t, [&]()
     {
         t->count_seqs_static(reinterpret_cast<const uint8_t*>(text), offsets, n, counts);
         return static_cast<int>(KACHE_OK);
@@ -3363,7 +3423,7 @@ int kache_amd_lookup_groups_device(const kache_table* t, const char* text, uint6
         return KACHE_EINVAL;
     if(n_groups == 0)
         return n == 0 ? static_cast<int>(KACHE_OK) : static_cast<int>(KACHE_EINVAL);
-    return guard([&]()
+    return guard(t, [&]()
     {
         std::vector<kache_table::LookupDest> d(n_groups);
         for(size_t i = 0; i < n_groups; ++i)
@@ -3386,7 +3446,7 @@ int kache_amd_lookup_groups(const kache_table* t, const char* const* seqs, const
         return KACHE_EINVAL;
     if(n_groups == 0)
         return n == 0 ? static_cast<int>(KACHE_OK) : static_cast<int>(KACHE_EINVAL);
-    return guard([&]()
+    return guard(t, [&]()
     {
         std::vector<kache_table::LookupDest> d(n_groups);
         for(size_t i = 0; i < n_groups; ++i)
@@ -3401,7 +3461,7 @@ int kache_amd_insert_seqs(kache_table* t, uint32_t user, const char* const* seqs
     if(t == nullptr || !registered(t, user) || !valid(seqs, lens, n) ||
        (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = t->insert_host(user, seqs, lens, n, t->map ? op : 0, false, t->map ? x : 0, 0);
         if(inserted != nullptr)
@@ -3414,7 +3474,7 @@ int kache_amd_update_seqs(kache_table* t, const char* const* seqs, const size_t*
 {
     if(t == nullptr || !valid(seqs, lens, n) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = t->update_host(seqs, lens, n, op, x);
         if(present != nullptr)
@@ -3428,7 +3488,7 @@ int kache_amd_insert_text(kache_table* t, uint32_t user, const char* text, uint6
     if(t == nullptr || !registered(t, user) || (text == nullptr && text_bytes > 0) ||
        (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = text_bytes ? t->insert_host(user, nullptr, nullptr, 0, t->map ? op : 0, false, t->map ? x : 0, 0, text, text_bytes) : 0;
         if(inserted != nullptr)
@@ -3441,7 +3501,7 @@ int kache_amd_update_text(kache_table* t, const char* text, uint64_t text_bytes,
 {
     if(t == nullptr || (text == nullptr && text_bytes > 0) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = text_bytes ? t->update_host(nullptr, nullptr, 0, op, x, text, text_bytes) : 0;
         if(present != nullptr)
@@ -3454,6 +3514,11 @@ int kache_amd_save(const kache_table* t, const char* path) noexcept
 {
     if(t == nullptr || path == nullptr)
         return KACHE_EINVAL;
+    if(t->damaged.load(std::memory_order_acquire))
+    {
+        last_error = t->damage;
+        return KACHE_EINTERNAL;
+    }
     try
     {
         t->save(path);
@@ -3521,14 +3586,14 @@ int kache_amd_histogram(const kache_table* t, uint64_t max, uint64_t* out) noexc
 {
     if(t == nullptr || out == nullptr || t->value_bytes == 0 || max > (uint64_t(1) << 30))
         return KACHE_EINVAL;
-    return guard([&](){ t->histogram(max, out); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->histogram(max, out); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_get(const kache_table* t, const char* kmer, uint8_t* found, uint64_t* value) noexcept
 {
     if(t == nullptr || kmer == nullptr || found == nullptr)
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         // The k characters at `kmer`, looked up as a one-position sequence
         // (include/Streaming_Kmer_Hash_Table.hpp:2607-2645).
@@ -3547,28 +3612,28 @@ int kache_size(const kache_table* t, uint64_t* keys) noexcept
 {
     if(t == nullptr || keys == nullptr)
         return KACHE_EINVAL;
-    return guard([&](){ std::lock_guard<std::mutex> lock(t->mu); *keys = t->size; return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ std::lock_guard<std::mutex> lock(t->mu); *keys = t->size; return static_cast<int>(KACHE_OK); });
 }
 
 int kache_reduce(const kache_table* t, uint32_t what, uint64_t mask, uint32_t workers, uint64_t* out) noexcept
 {
     if(t == nullptr || out == nullptr || workers < 1 || what > KACHE_REDUCE_SUM || (t->value_bytes == 0 && what != KACHE_REDUCE_KEYS))
         return KACHE_EINVAL;
-    return guard([&](){ *out = t->reduce(what, mask & t->vmax()); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ *out = t->reduce(what, mask & t->vmax()); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_for_each(const kache_table* t, uint32_t shards, uint32_t shard, kache_keys_fn f, void* ctx) noexcept
 {
     if(t == nullptr || f == nullptr || shard >= shards)
         return KACHE_EINVAL;
-    return guard([&](){ t->for_each(shards, shard, f, ctx); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->for_each(shards, shard, f, ctx); return static_cast<int>(KACHE_OK); });
 }
 
 int kache_get_stats(const kache_table* t, kache_stats* out) noexcept
 {
     if(t == nullptr || out == nullptr)
         return KACHE_EINVAL;
-    return guard([&](){ t->stats(*out); return static_cast<int>(KACHE_OK); });
+    return guard(t, [&](){ t->stats(*out); return static_cast<int>(KACHE_OK); });
 }
 
 const char* kache_strerror(int code) noexcept
@@ -3592,7 +3657,7 @@ int kache_amd_insert_device(kache_table* t, uint32_t user, const char* text, uin
     if(t == nullptr || !registered(t, user) || !device_text_ok(text, text_bytes) ||
        (t->map && ((op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = text_bytes ? t->insert_device(user, reinterpret_cast<const uint8_t*>(text), text_bytes, t->map ? op : 0, false, t->map ? x : 0, 0) : 0;
         if(inserted != nullptr)
@@ -3606,7 +3671,7 @@ int kache_amd_insert_oriented_device(kache_table* t, uint32_t user, const char* 
     const bool op_ok = (op == KACHE_ASSIGN || op == KACHE_ADD || op == KACHE_OR || op == KACHE_MAX);
     if(t == nullptr || !registered(t, user) || !device_text_ok(text, text_bytes) || !op_ok || !fits(t, x_fwd) || !fits(t, x_rev))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         const uint64_t c = text_bytes ? t->insert_device(user, reinterpret_cast<const uint8_t*>(text), text_bytes, op, true, x_fwd, x_rev) : 0;
         if(inserted != nullptr)
@@ -3624,7 +3689,7 @@ int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, co
     for(uint64_t i = 0; i < n_batches; ++i)
         if(bounds[i] > bounds[i + 1])
             return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         t->insert_batches(user, reinterpret_cast<const uint8_t*>(text), bounds, n_batches, t->map ? op : 0, false, t->map ? x : 0, 0, inserted);
         return static_cast<int>(KACHE_OK);
@@ -3638,7 +3703,7 @@ int kache_amd_count_batches(const kache_table* t, const char* text, const uint64
     for(uint64_t i = 0; i < n_batches; ++i)
         if(bounds[i] > bounds[i + 1])
             return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         if(n_batches)
             t->count_batches(reinterpret_cast<const uint8_t*>(text), bounds, n_batches, hits, positions);
@@ -3650,7 +3715,7 @@ int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_byte
 {
     if(t == nullptr || !device_text_ok(text, text_bytes) || (op > KACHE_MAX && op != KACHE_AMD_ADD_WRAP) || !fits(t, x))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         uint64_t c = 0;
         if(text_bytes)
@@ -3668,7 +3733,7 @@ int kache_amd_count_device(const kache_table* t, const char* text, uint64_t text
 {
     if(t == nullptr || hits == nullptr || !device_text_ok(text, text_bytes))
         return KACHE_EINVAL;
-    return guard([&]()
+    return guard(t, [&]()
     {
         Counters c{};
         if(text_bytes)

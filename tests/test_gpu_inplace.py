@@ -236,3 +236,83 @@ def test_budget_returns_room_a_kept_range_brought(api):
     assert np.array_equal(k, ko) and np.array_equal(v, vo)
     assert (t.count(queries) == ot.counts(queries)[0]).all()
     t.close()
+
+
+
+def test_a_doubling_that_fails_part_way(api, monkeypatch):
+    """A doubling whose away list cannot grow (refused from the host through
+    KACHE_AMD_TEST_FAIL_AWAY_REGROW, as if the device had no memory left).
+    Out of place the call fails with KACHE_ENOMEM and the table stays as it
+    was: the same calls again give the oracle's keys.  In place the split has
+    overwritten old buckets: the call fails and every later call on the
+    table's keys is refused with KACHE_EINTERNAL; its info and release still
+    work, and a new table is unaffected."""
+    g = G.genome(300_000, 361)
+    reads = G.reads(g, 24_000, 150, 362, sub=0.01, nrate=0.001)
+    queries = G.reads(g, 300, 150, 363, sub=0.01)
+    chunks = [reads[i:i + 2000] for i in range(0, len(reads), 2000)]
+    print("a doubling that fails part way: out of place", flush=True)
+    t = capi.Table(api, 0, expected_keys=8_000)
+    t.set_growth(capi.GROW_OUT_OF_PLACE, 256)   # a list of 256 entries always has to grow
+    u = t.register()
+    ot = O.OTable(0)
+    monkeypatch.setenv("KACHE_AMD_TEST_FAIL_AWAY_REGROW", "1")
+    with pytest.raises(capi.KacheError) as e:
+        t.insert(chunks[0], capi.KEEP, 0, u)
+    monkeypatch.delenv("KACHE_AMD_TEST_FAIL_AWAY_REGROW")
+    assert e.value.status == capi.KACHE_ENOMEM and "away list" in str(e.value), e.value
+    assert t.info()["resizes"] == 0
+    # The failed call's launches before the doubling stay applied; a set's
+    # inserts are idempotent, so the same calls again complete the keys.
+    for part in chunks[:4]:
+        t.insert(part, capi.KEEP, 0, u)
+        ot.insert(part, capi.KEEP, 0)
+    info = t.info()
+    assert info["resizes"] >= 2 and info["in_place_doublings"] == 0 and info["away_list_regrows"] >= 1, info
+    k, _, _ = t.for_each()
+    assert np.array_equal(k, ot.dump()[0])
+    assert (t.count(queries) == ot.counts(queries)[0]).all()
+    t.close()
+
+    print("a doubling that fails part way: in place", flush=True)
+    t = capi.Table(api, 0, expected_keys=8_000)
+    t.set_growth(capi.GROW_IN_PLACE, 256)
+    u = t.register()
+    # The small first store cannot grow in place: the first doubling moves it
+    # out of place into a range with room for more.
+    i = 0
+    while t.info()["resizes"] == 0:
+        t.insert(chunks[i], capi.KEEP, 0, u)
+        i += 1
+    info = t.info()
+    grain = 2 << 20
+    room = -(-2 * info["buckets"] * info["bucket_bytes"] // grain) * grain
+    assert info["store_reserved_bytes"] >= room, info          # so the next doubling runs in place
+    resizes, in_place = info["resizes"], info["in_place_doublings"]
+    monkeypatch.setenv("KACHE_AMD_TEST_FAIL_AWAY_REGROW", "1")
+    with pytest.raises(capi.KacheError) as e:
+        for part in chunks[i:]:
+            t.insert(part, capi.KEEP, 0, u)
+    monkeypatch.delenv("KACHE_AMD_TEST_FAIL_AWAY_REGROW")
+    assert e.value.status == capi.KACHE_ENOMEM, e.value
+    info = t.info()
+    assert info["resizes"] == resizes and info["in_place_doublings"] == in_place, info
+    for what, call in (("insert", lambda: t.insert(chunks[0], capi.KEEP, 0, u)), ("count", lambda: t.count(queries)),
+                       ("lookup", lambda: t.lookup(queries)), ("for_each", t.for_each), ("size", t.size),
+                       ("stats", t.stats)):
+        with pytest.raises(capi.KacheError) as e:
+            call()
+        assert e.value.status == capi.KACHE_EINTERNAL and "in-place doubling failed" in str(e.value), (what, e.value)
+    t.close()
+
+    # A new table, in place as well, is unaffected.
+    t = capi.Table(api, 0, expected_keys=8_000)
+    t.set_growth(capi.GROW_IN_PLACE)
+    u = t.register()
+    ot = O.OTable(0)
+    for part in chunks[:6]:
+        assert t.insert(part, capi.KEEP, 0, u) == ot.insert(part, capi.KEEP, 0)
+    assert t.info()["in_place_doublings"] >= 1
+    k, _, _ = t.for_each()
+    assert np.array_equal(k, ot.dump()[0])
+    t.close()
