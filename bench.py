@@ -40,6 +40,8 @@ LINES_PER_KMER = 1 / 5.29 + 16 / 64
 
 # Timed passes whose launches are bracketed by HIP events: one in PROFILE_EVERY.
 PROFILE_EVERY = 4
+# Timed lookup passes over the lookup reads (the median is reported).
+LOOKUP_PASSES = 5
 GROWTH = {"auto": 0, "in-place": 1, "out-of-place": 2}    # KACHE_AMD_GROW_*
 
 
@@ -303,16 +305,23 @@ def main():
     timed_kmers = kmers_per_pass * steps
 
     # Lookups: 2M fresh reads, in batches of 65,536, back to back, on the
-    # last pass's table.
+    # last pass's table.  LOOKUP_PASSES timed passes without kernel timing
+    # events (the median pass is reported), then one profiled pass for the
+    # query kernel's own time.
     lb = np.unique(np.array([min(r, a.lookup_reads) * L1 for r in range(0, a.lookup_reads + a.batch, a.batch)], np.uint64))
     t.count_batches(look.ptr, lb)       # untimed warm-up of the query path
-    t.profile(True)
-    t1 = time.perf_counter()
-    h, p = t.count_batches(look.ptr, lb)
-    dt_look = time.perf_counter() - t1
+    look_dts = []
+    for _ in range(LOOKUP_PASSES):
+        t1 = time.perf_counter()
+        h, p = t.count_batches(look.ptr, lb)
+        look_dts.append(time.perf_counter() - t1)
+    dt_look = float(np.median(look_dts))
     hits, look_kmers = int(h.sum()), int(p.sum())
+    t.profile(True)
+    h2, _ = t.count_batches(look.ptr, lb)
     q_ms, q_launches = t.kernel_time("query")
     t.profile(False)
+    assert int(h2.sum()) == hits, "lookup passes disagree"
 
     per_launch_kmers = kmers_per_pass * profiled / max(ins_launches, 1)
     avg_ms = ins_ms / max(ins_launches, 1)
@@ -355,7 +364,7 @@ def main():
                    "reads": n_reads, "batches": n_batches, "batch_reads": a.batch, "expected_keys": a.expected,
                    "genome_bp": a.genome, "growth": a.growth},
         "lookups_per_s": look_kmers / dt_look,
-        "lookup_hits": hits, "lookup_kmers": look_kmers,
+        "lookup_hits": hits, "lookup_kmers": look_kmers, "lookup_pass_ms": [x * 1e3 for x in look_dts],
         "timed_kmers": timed_kmers, "kmers_per_step": kmers_per_pass,
         "keys": stats["keys"], "capacity": stats["capacity"], "resizes": info["resizes"],
         "in_place_doublings": info["in_place_doublings"], "store_mapped_bytes": info["store_mapped_bytes"],

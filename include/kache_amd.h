@@ -35,8 +35,10 @@ int kache_amd_update_device(kache_table* t, const char* text, uint64_t text_byte
  * new keys at inserted[i], unless `inserted` is NULL), with the launches
  * pipelined: a batch starts before the previous one's count is read
  * whenever no doubling can fall due in between.  kache_amd_count_batches
- * looks them up back to back: present positions at hits[i] and, unless
- * NULL, all k-mer positions at positions[i]. */
+ * looks them up: present positions at hits[i] and, unless NULL, all k-mer
+ * positions at positions[i], a k-mer counting only if it lies wholly within
+ * its batch (so the counts are those of one kache_amd_count_device call per
+ * batch); up to 256 batches run in one launch. */
 int kache_amd_insert_batches(kache_table* t, uint32_t user, const char* text, const uint64_t* bounds, uint64_t n_batches, uint32_t op, uint64_t x, uint64_t* inserted) KACHE_NOEXCEPT;
 int kache_amd_count_batches(const kache_table* t, const char* text, const uint64_t* bounds, uint64_t n_batches, uint64_t* hits, uint64_t* positions) KACHE_NOEXCEPT;
 

@@ -275,6 +275,121 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
     }
 }
 
+// The number of batch bounds bounds[1..n] at or below p (p the same on
+// every lane): the batch of position p, or n past the last one.  One load
+// per lane per 64 bounds.
+__device__ __forceinline__ uint32_t bounds_at_or_below(const uint64_t* bounds, uint32_t n, uint64_t p)
+{
+    uint32_t c = 0;
+    for(uint32_t base = 1; base <= n; base += 64)
+    {
+        const uint32_t i = base + __lane_id();
+        c += __popcll(__ballot(i <= n && bounds[i] <= p));
+    }
+    return c;
+}
+
+// Lookups of the batches [bounds[i], bounds[i + 1]) of a device text in one
+// launch (kache_amd_count_batches): k-mer positions and hits per batch, a
+// k-mer counting only if it ends within its batch, as a launch per batch
+// counts it.  A wave's tiles go up the text, so it leaves a batch for good:
+// it sums its positions of the batch it is in, lane by lane in registers,
+// and stores the sums into its own pair of part (n x waves pairs of u32
+// hits, positions, zeroed before) when it leaves.  A wave tile that
+// straddles batch bounds adds into edge (n pairs of u64) by wave-aggregated
+// atomics.  k_batch_sums adds the two up.
+template <int VB>
+__global__ KAMD_STREAM_BOUNDS void k_count_batches(TableView T, const uint8_t* __restrict__ text, const uint64_t* __restrict__ bounds, uint32_t n,
+                                                   uint32_t* __restrict__ part, uint64_t* __restrict__ edge)
+{
+    __shared__ WaveLds wave_lds[TILE / 64];
+    WaveLds& s = wave_lds[threadIdx.x >> 6];
+    const uint32_t lane = __lane_id();
+    const uint64_t waves = uint64_t(gridDim.x) * (TILE / 64);
+    const uint64_t wave = uint64_t(blockIdx.x) * (TILE / 64) + (threadIdx.x >> 6);
+    const uint64_t origin = bounds[0] & ~uint64_t(15);
+    uint32_t cur = n;               // the batch summed in registers (n: none)
+    uint64_t ce = 0;                // its end (the wave's positions only grow: past its start)
+    uint32_t lh = 0, lp = 0;        // this lane's hits and positions in it
+    const auto flush = [&]
+    {
+        const uint32_t h = static_cast<uint32_t>(wave_sum(lh)), q = static_cast<uint32_t>(wave_sum(lp));
+        if(cur < n && lane == 0)
+        {
+            uint32_t* dst = part + 2 * (uint64_t(cur) * waves + wave);
+            dst[0] = h, dst[1] = q;
+        }
+        lh = 0, lp = 0;
+    };
+    for_tiles(text, bounds[0], bounds[n], bounds[n], s, [&](uint64_t tile, const Desc& d)
+    {
+        const uint64_t p = origin + tile * TILE + threadIdx.x;
+        const uint64_t p0 = p - lane;
+        // The end of this lane's batch; `edge_tile` when bounds fall inside
+        // the wave tile (or it lies past the last batch): each lane in its
+        // own batch `bl`, sums added by atomics.
+        uint64_t end = ce;
+        uint32_t bl = cur;
+        bool edge_tile = false;
+        if(p0 + 63 >= ce)
+        {
+            const uint32_t b0 = bounds_at_or_below(bounds, n, p0), b63 = bounds_at_or_below(bounds, n, p0 + 63);
+            if(b0 != b63 || b0 >= n)
+            {
+                edge_tile = true;
+                bl = b0;
+                while(bl < b63 && bounds[bl + 1] <= p)
+                    bl++;
+                end = bl < n ? bounds[bl + 1] : 0;
+            }
+            else
+            {
+                if(b0 != cur)
+                    flush();
+                cur = b0, ce = bounds[b0 + 1], end = ce, bl = b0;
+            }
+        }
+        const bool v = d.valid && p + K <= end;
+        const bool hit = v && find_key<VB>(T, d.key, d.h, d.c).k != nullptr;
+        if(!edge_tile)
+        {
+            lp += v, lh += hit;
+            return;
+        }
+        uint64_t pending = __ballot(v);
+        while(pending)
+        {
+            const uint32_t b = __shfl(bl, __builtin_ctzll(pending), 64);
+            const bool mine = v && bl == b;
+            const uint64_t sh = wave_sum(mine && hit), sp = wave_sum(mine);
+            if(lane == 0)
+                add64(edge + 2 * b, sh), add64(edge + 2 * b + 1, sp);
+            pending &= ~__ballot(mine);
+        }
+    });
+    flush();
+}
+
+// Sums of k_count_batches: block b adds batch b's pairs of part and edge
+// into out[2b] (hits) and out[2b + 1] (positions).
+__global__ __launch_bounds__(256) void k_batch_sums(const uint32_t* __restrict__ part, uint64_t waves, const uint64_t* __restrict__ edge, uint64_t* out)
+{
+    __shared__ uint64_t red[2][4];
+    const uint64_t b = blockIdx.x;
+    uint64_t h = 0, q = 0;
+    for(uint64_t w = threadIdx.x; w < waves; w += blockDim.x)
+        h += part[2 * (b * waves + w)], q += part[2 * (b * waves + w) + 1];
+    h = wave_sum(h), q = wave_sum(q);
+    if((threadIdx.x & 63) == 0)
+        red[0][threadIdx.x >> 6] = h, red[1][threadIdx.x >> 6] = q;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        out[2 * b] = edge[2 * b] + red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        out[2 * b + 1] = edge[2 * b + 1] + red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Resize: doubling (include/Streaming_Kmer_Hash_Table.hpp:1473-1681).  Old
 // bucket b's primaries move to new buckets 2b and 2b + 1, in their slot

@@ -374,3 +374,45 @@ def test_pipelined_batches_equal_one_call_per_batch(api, expected):
             sub, m = capi.device_text(api, [text[lo:hi]])
             h2, p2 = b.count_device(sub.ptr, m, positions=True)
             assert (int(hits[i]), int(pos[i])) == (h2, p2) and h2 == p2
+
+
+def test_count_batches_with_bounds_anywhere(api):
+    """kache_amd_count_batches runs up to 256 batches in one launch; each
+    batch's hits and k-mer positions must equal the oracle's count of its
+    own slice of the text (a k-mer counts only within one batch), with bounds
+    at arbitrary byte offsets (k-mers cut by them), empty batches, batches
+    shorter than a k-mer or a 64-position wave, a first bound neither 0 nor
+    16-aligned, and several launch groups (more than 256 batches)."""
+    g = G.genome(400_000, 91)
+    R = G.reads(g, 20_000, 150, 92, sub=0.01, nrate=0.001, lower=0.2)
+    Q = G.reads(g, 6_000, 150, 93, sub=0.02, nrate=0.002, lower=0.2) + G.pieces(G.genome(20_000, 94), 300)
+    text = b"".join(q + b"\n" for q in Q)
+    rng = np.random.default_rng(95)
+    start = 13
+    cuts = rng.integers(start, len(text), 600)
+    tiny = start + 50_000 + np.cumsum(rng.integers(0, 40, 200))
+    bounds = np.sort(np.concatenate([[start, len(text)], cuts, cuts[:20], tiny])).astype(np.uint64)
+    print(f"count_batches: {len(bounds) - 1} batches over {len(text)} bytes", flush=True)
+    arr = np.frombuffer(text, np.uint8)
+    base = np.isin(arr | 0x20, np.frombuffer(b"acgt", np.uint8)).astype(np.int64)
+    run = np.concatenate([[0], np.cumsum(base)])
+    ok = np.zeros(len(arr) + 1, np.int64)
+    ok[:len(arr) - 30] = (run[31:] - run[:-31]) == 31            # a whole k-mer starts at p
+    P = np.concatenate([[0], np.cumsum(ok)])
+    t = capi.Table(api, 8, expected_keys=1_000_000)
+    ot = O.OTable(8)
+    u = t.register()
+    assert t.insert(R, capi.ADD, 1, u) == ot.insert(R, capi.ADD, 1)
+    buf, _ = capi.device_text(api, [text])
+    hits, pos = t.count_batches(buf.ptr, bounds)
+    for i in range(len(bounds) - 1):
+        lo, hi = int(bounds[i]), int(bounds[i + 1])
+        want_p = int(P[hi - 30] - P[lo]) if hi - lo >= 31 else 0
+        want_h = int(ot.counts([text[lo:hi]])[0][0]) if hi > lo else 0
+        assert (int(hits[i]), int(pos[i])) == (want_h, want_p), (i, lo, hi)
+    assert int(pos.sum()) > 0 and 0 < int(hits.sum()) < int(pos.sum())
+    h1, p1 = t.count_batches(buf.ptr, np.array([0, len(text)], np.uint64))
+    assert int(h1[0]) == int(ot.counts([text])[0][0]) and int(p1[0]) == int(P[len(text) - 30])
+    h0, p0 = t.count_batches(buf.ptr, np.array([5, 5, 5], np.uint64))
+    assert h0.tolist() == [0, 0] and p0.tolist() == [0, 0]
+    t.close()
