@@ -53,9 +53,26 @@ struct DeferRing
     uint32_t cf[DEFER_RING];   // checksum | forward << 8
 };
 
+// One wave (all lanes): adds up counter set `src` (lane l reads copy l) into
+// `dst` (host memory: CNT_WORDS words) and zeroes the set.
+__device__ __forceinline__ void publish_set(uint64_t* src, uint64_t* dst)
+{
+    static_assert(CNT_STRIPES == 64, "one lane per copy");
+    const uint32_t lane = __lane_id();
+    uint64_t* mine = src + uint64_t(lane) * CNT_STRIPE_WORDS;
+    for(uint32_t i = 0; i < CNT_WORDS; ++i)
+    {
+        const uint64_t v = wave_sum(mine[i]);
+        mine[i] = 0;
+        if(lane == 0)
+            __hip_atomic_store(dst + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // The counters of the previous launch of a pipeline, handed to the host by
-// the next launch: `src` (complete, since launches on a stream run in order)
-// is copied to host memory and cleared, then `flag` is set to `seq`.
+// the next launch: set `src` (complete, since launches on a stream run in
+// order) is added up into host memory and cleared, then `flag` is set to
+// `seq`.
 struct Handoff
 {
     uint64_t* src;
@@ -66,14 +83,11 @@ struct Handoff
 
 __device__ __forceinline__ void hand_off(const Handoff& h)
 {
-    if(h.src && blockIdx.x == 0 && threadIdx.x == 0)
+    if(h.src && blockIdx.x == 0 && threadIdx.x < 64)
     {
-        for(uint32_t i = 0; i < sizeof(Counters) / 8; ++i)
-        {
-            __hip_atomic_store(h.dst + i, h.src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            h.src[i] = 0;
-        }
-        __hip_atomic_store(h.flag, h.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        publish_set(h.src, h.dst);
+        if(threadIdx.x == 0)
+            __hip_atomic_store(h.flag, h.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -140,9 +154,9 @@ __global__ __launch_bounds__(BT, WAVES) void k_insert(TableView T, const uint8_t
         wave_sync();
         settle(lane < queued, (head + lane) % DEFER_RING);
     }
-    block_add(lane == 0 ? ins : 0, &cnt->inserted, red);
-    block_add(lane == 0 ? ins_ov : 0, &cnt->ov_inserted, red);
-    block_add(lane == 0 ? failed : 0, &cnt->failed, red);
+    block_add(lane == 0 ? ins : 0, &stripe_of(cnt)->inserted, red);
+    block_add(lane == 0 ? ins_ov : 0, &stripe_of(cnt)->ov_inserted, red);
+    block_add(lane == 0 ? failed : 0, &stripe_of(cnt)->failed, red);
 }
 
 // Present-only updates (include/Streaming_Kmer_Hash_Table.hpp:2648-2691).
@@ -162,7 +176,7 @@ __global__ KAMD_STREAM_BOUNDS void k_update(TableView T, const uint8_t* __restri
                 value_op<OP>(slot.v(), x, T.vmax, false);
             }
     });
-    block_add(present, &cnt->present, red);
+    block_add(present, &stripe_of(cnt)->present, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,8 +284,8 @@ __global__ KAMD_STREAM_BOUNDS void k_query(TableView T, const uint8_t* __restric
     });
     if constexpr(MODE == Q_TOTAL)
     {
-        block_add(total, &cnt->present, red);
-        block_add(positions, &cnt->positions, red);
+        block_add(total, &stripe_of(cnt)->present, red);
+        block_add(positions, &stripe_of(cnt)->positions, red);
     }
 }
 
@@ -864,16 +878,13 @@ __global__ void k_zero64(uint64_t* p, uint64_t n)
         p[i] = 0;
 }
 
-// Moves n u64 words from `src` (device) to `dst` (host memory) and zeroes
-// `src`: a launch's counters read back and cleared for the next launch by
-// one dispatch.
-__global__ void k_publish(uint64_t* src, uint64_t* dst, uint64_t n)
+// Adds up `sets` consecutive counter sets from `src` (device) into as many
+// Counters at `dst` (host memory) and zeroes them: a launch's counters read
+// back and cleared for the next launch by one dispatch of one wave.
+__global__ __launch_bounds__(64) void k_publish(uint64_t* src, uint64_t* dst, uint64_t sets)
 {
-    for(uint64_t i = threadIdx.x; i < n; i += blockDim.x)
-    {
-        __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        src[i] = 0;
-    }
+    for(uint64_t k = 0; k < sets; ++k)
+        publish_set(src + k * CNT_SET_WORDS, dst + k * CNT_WORDS);
 }
 
 // Describes the valid positions of [pos_begin, pos_end) into arrays (for

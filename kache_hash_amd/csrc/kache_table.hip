@@ -901,7 +901,7 @@ struct kache_table
         }
 
         dscratch = &device_scratch(device);
-        d_cnt = dev_alloc<Counters>(1);
+        d_cnt = reinterpret_cast<Counters*>(dev_alloc<uint64_t>(CNT_SET_WORDS));
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_cnt), sizeof(Counters), hipHostMallocDefault));
         zero_counters();
         try
@@ -1093,16 +1093,15 @@ struct kache_table
 
     void zero_counters() const
     {
-        hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), sizeof(Counters) / 8);
+        hipLaunchKernelGGL(k_zero64, dim3(4), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), CNT_SET_WORDS);
         HIP(hipGetLastError());
     }
 
     // Enqueues the read-back of `n` counter sets of the launches so far into
-    // `out` (host memory) and their zeroing for the next launches.
+    // `out` (host memory, n Counters) and their zeroing for the next launches.
     void publish_counters(Counters* dev, Counters* out, uint64_t n = 1) const
     {
-        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(dev), reinterpret_cast<uint64_t*>(out),
-                           n * sizeof(Counters) / 8);
+        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(dev), reinterpret_cast<uint64_t*>(out), n);
         HIP(hipGetLastError());
     }
 
@@ -2002,7 +2001,7 @@ struct kache_table
             if(le == hipSuccess)
             {
                 hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t*>(d_cnt), reinterpret_cast<uint64_t*>(h_cnt),
-                                   sizeof(Counters) / 8);
+                                   uint64_t(1));
                 le = hipGetLastError();
             }
             hipGraph_t graph = nullptr;
@@ -2388,7 +2387,7 @@ struct kache_table
 
         const auto publish = [&](Flight& f)
         {
-            publish_counters(d_ring + f.slot, h_ring + f.slot);
+            publish_counters(ring_set(f.slot), h_ring + f.slot);
             HIP(hipEventRecord(ring_ev[f.slot], stream));
             f.ev = f.slot;
         };
@@ -2473,9 +2472,9 @@ struct kache_table
                 {
                     Flight& p = q.back();
                     p.seq = ++ring_seq;
-                    prev = {reinterpret_cast<uint64_t*>(d_ring + p.slot), reinterpret_cast<uint64_t*>(h_ring + p.slot), h_seq + p.slot, p.seq};
+                    prev = {reinterpret_cast<uint64_t*>(ring_set(p.slot)), reinterpret_cast<uint64_t*>(h_ring + p.slot), h_seq + p.slot, p.seq};
                 }
-                timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, d_ring + slot, prev); });
+                timed("insert", [&]{ dispatch_insert(op, oriented, text, e, b, e, x, xr, ring_set(slot), prev); });
                 q.push_back({i, slot, pos, -1, 0});
                 in_flight += pos;
                 const uint64_t next = i + 1 < n ? bounds[i + 2] - bounds[i + 1] : 0;
@@ -2495,7 +2494,7 @@ struct kache_table
             // clear the ring for the next call (it is otherwise zero whenever
             // no launch is in flight).
             (void)hipStreamSynchronize(stream);
-            hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * sizeof(Counters) / 8);
+            hipLaunchKernelGGL(k_zero64, dim3(16), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * CNT_SET_WORDS);
             (void)hipStreamSynchronize(stream);
             throw;
         }
@@ -2563,7 +2562,8 @@ struct kache_table
     }
 
     static constexpr int RING = 16;         // counter slots of batches in flight
-    mutable Counters* d_ring = nullptr;
+    mutable Counters* d_ring = nullptr;     // RING counter sets (CNT_SET_WORDS words each)
+    Counters* ring_set(uint64_t slot) const { return reinterpret_cast<Counters*>(reinterpret_cast<uint64_t*>(d_ring) + slot * CNT_SET_WORDS); }
     mutable Counters* h_ring = nullptr;
     mutable uint64_t* h_seq = nullptr;      // per slot: the last hand-off's sequence number
     mutable uint64_t ring_seq = 0;
@@ -2573,13 +2573,13 @@ struct kache_table
     {
         if(d_ring)
             return;
-        d_ring = dev_alloc<Counters>(RING);
+        d_ring = reinterpret_cast<Counters*>(dev_alloc<uint64_t>(RING * CNT_SET_WORDS));
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ring), RING * sizeof(Counters), hipHostMallocDefault));
         HIP(hipHostMalloc(reinterpret_cast<void**>(&h_seq), RING * sizeof(uint64_t), hipHostMallocDefault));
         std::memset(h_seq, 0, RING * sizeof(uint64_t));
         for(auto& e : ring_ev)
             HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * sizeof(Counters) / 8);
+        hipLaunchKernelGGL(k_zero64, dim3(16), dim3(256), 0, stream, reinterpret_cast<uint64_t*>(d_ring), RING * CNT_SET_WORDS);
         HIP(hipGetLastError());
     }
 
