@@ -306,15 +306,18 @@ __device__ __forceinline__ uint32_t bounds_at_or_below(const uint64_t* bounds, u
 // Lookups of the batches [bounds[i], bounds[i + 1]) of a device text in one
 // launch (kache_amd_count_batches): k-mer positions and hits per batch, a
 // k-mer counting only if it ends within its batch, as a launch per batch
-// counts it.  A wave's tiles go up the text, so it leaves a batch for good:
-// it sums its positions of the batch it is in, lane by lane in registers,
-// and stores the sums into its own pair of part (n x waves pairs of u32
-// hits, positions, zeroed before) when it leaves.  A wave tile that
-// straddles batch bounds adds into edge (n pairs of u64) by wave-aggregated
-// atomics.  k_batch_sums adds the two up.
+// counts it.  This kernel counts the wave tiles (64 positions from p0) with
+// no batch bound in (p0, p0 + 93], whose k-mers all lie in one batch; the
+// others, the edge tiles (at most two per bound), are k_count_edges'.  A
+// wave's tiles go up the text, so it leaves a batch for good: it sums its
+// positions of the batch it is in, lane by lane in registers, and stores the
+// sums into its own pair of part (n x waves pairs of u32 hits, positions,
+// zeroed before) when it leaves.  k_batch_sums adds part and edge up.
+constexpr uint64_t EDGE_REACH = 63 + K - 1;     // a bound within p0 + 1 .. p0 + EDGE_REACH makes an edge tile
+
 template <int VB>
 __global__ KAMD_STREAM_BOUNDS void k_count_batches(TableView T, const uint8_t* __restrict__ text, const uint64_t* __restrict__ bounds, uint32_t n,
-                                                   uint32_t* __restrict__ part, uint64_t* __restrict__ edge)
+                                                   uint32_t* __restrict__ part)
 {
     __shared__ WaveLds wave_lds[TILE / 64];
     WaveLds& s = wave_lds[threadIdx.x >> 6];
@@ -323,7 +326,7 @@ __global__ KAMD_STREAM_BOUNDS void k_count_batches(TableView T, const uint8_t* _
     const uint64_t wave = uint64_t(blockIdx.x) * (TILE / 64) + (threadIdx.x >> 6);
     const uint64_t origin = bounds[0] & ~uint64_t(15);
     uint32_t cur = n;               // the batch summed in registers (n: none)
-    uint64_t ce = 0;                // its end (the wave's positions only grow: past its start)
+    uint64_t ce = 0;                // its end (the wave's tiles only go up: past its start)
     uint32_t lh = 0, lp = 0;        // this lane's hits and positions in it
     const auto flush = [&]
     {
@@ -337,39 +340,48 @@ __global__ KAMD_STREAM_BOUNDS void k_count_batches(TableView T, const uint8_t* _
     };
     for_tiles(text, bounds[0], bounds[n], bounds[n], s, [&](uint64_t tile, const Desc& d)
     {
-        const uint64_t p = origin + tile * TILE + threadIdx.x;
-        const uint64_t p0 = p - lane;
-        // The end of this lane's batch; `edge_tile` when bounds fall inside
-        // the wave tile (or it lies past the last batch): each lane in its
-        // own batch `bl`, sums added by atomics.
-        uint64_t end = ce;
-        uint32_t bl = cur;
-        bool edge_tile = false;
-        if(p0 + 63 >= ce)
+        const uint64_t p0 = origin + tile * TILE + (threadIdx.x & ~63u);
+        if(p0 + EDGE_REACH >= ce)
         {
-            const uint32_t b0 = bounds_at_or_below(bounds, n, p0), b63 = bounds_at_or_below(bounds, n, p0 + 63);
-            if(b0 != b63 || b0 >= n)
+            const uint32_t b0 = bounds_at_or_below(bounds, n, p0);
+            if(b0 >= n || p0 + EDGE_REACH >= bounds[b0 + 1])
+                return;         // an edge tile (or past the last batch)
+            if(b0 != cur)
             {
-                edge_tile = true;
-                bl = b0;
-                while(bl < b63 && bounds[bl + 1] <= p)
-                    bl++;
-                end = bl < n ? bounds[bl + 1] : 0;
-            }
-            else
-            {
-                if(b0 != cur)
-                    flush();
-                cur = b0, ce = bounds[b0 + 1], end = ce, bl = b0;
+                flush();
+                cur = b0, ce = bounds[b0 + 1];
             }
         }
-        const bool v = d.valid && p + K <= end;
+        lp += d.valid;
+        if(d.valid && find_key<VB>(T, d.key, d.h, d.c).k)
+            lh++;
+    });
+    flush();
+}
+
+// The edge tiles of k_count_batches: wave unit tiles[i] (positions origin +
+// 64 tiles[i] onwards, origin = bounds[0] rounded down to 16), one wave per
+// tile at a time; each lane counts its position in its own batch, a k-mer
+// only if it ends within that batch, into edge (n pairs of u64 hits,
+// positions) by wave-aggregated atomics.
+template <int VB>
+__global__ __launch_bounds__(64) void k_count_edges(TableView T, const uint8_t* __restrict__ text, const uint64_t* __restrict__ bounds, uint32_t n,
+                                                    const uint64_t* __restrict__ tiles, uint64_t n_tiles, uint64_t* __restrict__ edge)
+{
+    __shared__ WaveLds s;
+    const uint32_t lane = __lane_id();
+    const uint64_t origin = bounds[0] & ~uint64_t(15);
+    for(uint64_t i = blockIdx.x; i < n_tiles; i += gridDim.x)
+    {
+        const uint64_t p0 = origin + tiles[i] * 64, p = p0 + lane;
+        uint2 w{};
+        load_wave_tile(text, p0, w);
+        const Desc d = describe_wave(text, p0, bounds[n], bounds[n], s, w, 0, false);
+        uint32_t bl = bounds_at_or_below(bounds, n, p0);
+        while(bl < n && bounds[bl + 1] <= p)
+            bl++;
+        const bool v = d.valid && p >= bounds[0] && bl < n && p + K <= bounds[bl + 1];
         const bool hit = v && find_key<VB>(T, d.key, d.h, d.c).k != nullptr;
-        if(!edge_tile)
-        {
-            lp += v, lh += hit;
-            return;
-        }
         uint64_t pending = __ballot(v);
         while(pending)
         {
@@ -380,8 +392,7 @@ __global__ KAMD_STREAM_BOUNDS void k_count_batches(TableView T, const uint8_t* _
                 add64(edge + 2 * b, sh), add64(edge + 2 * b + 1, sp);
             pending &= ~__ballot(mine);
         }
-    });
-    flush();
+    }
 }
 
 // Sums of k_count_batches: block b adds batch b's pairs of part and edge

@@ -2519,30 +2519,52 @@ struct kache_table
         });
         const uint64_t waves = uint64_t(grid) * (TILE / 64);
         const uint64_t g = std::min(n, COUNT_GROUP);
-        // Per group: its n + 1 bounds, the per-wave pairs (u32), the edge
-        // sums and the results (u64 pairs); results of all groups after them.
-        const uint64_t bounds_w = (COUNT_GROUP + 1 + 1) & ~uint64_t(1), part_w = g * waves, edge_w = 2 * g;
-        d_bcnt.reserve((bounds_w + part_w + edge_w + 2 * n) * 8);
+        // Per group: its n + 1 bounds and its edge tiles (at most two per
+        // bound), the per-wave pairs (u32), the edge sums and the results
+        // (u64 pairs); results of all groups after them.
+        const uint64_t bounds_w = COUNT_GROUP + 2, tiles_w = 2 * COUNT_GROUP, part_w = g * waves, edge_w = 2 * g;
+        d_bcnt.reserve((bounds_w + tiles_w + part_w + edge_w + 2 * n) * 8);
         uint64_t* d_bounds = d_bcnt.as<uint64_t>();
-        uint32_t* part = reinterpret_cast<uint32_t*>(d_bounds + bounds_w);
-        uint64_t* edge = d_bounds + bounds_w + part_w;
+        uint64_t* d_tiles = d_bounds + bounds_w;
+        uint32_t* part = reinterpret_cast<uint32_t*>(d_tiles + tiles_w);
+        uint64_t* edge = d_tiles + tiles_w + part_w;
         uint64_t* out = edge + edge_w;
+        std::vector<uint64_t> staged(bounds_w + tiles_w);
         for(uint64_t g0 = 0; g0 < n; g0 += COUNT_GROUP)
         {
             const uint64_t m = std::min(COUNT_GROUP, n - g0);
-            HIP(hipMemcpyAsync(d_bounds, bounds + g0, (m + 1) * 8, hipMemcpyHostToDevice, stream));
+            // Edge tiles: wave units (64 positions from origin + 64 u) with a
+            // bound of bounds[g0 + 1 .. g0 + m] in (start, start + EDGE_REACH].
+            const uint64_t origin = bounds[g0] & ~uint64_t(15);
+            uint64_t n_edges = 0;
+            for(uint64_t i = 1; i <= m; ++i)
+            {
+                const uint64_t B = bounds[g0 + i];
+                if(B <= origin)
+                    continue;
+                const uint64_t first = B - origin > EDGE_REACH ? (B - origin - EDGE_REACH + 63) / 64 : 0, last = (B - origin - 1) / 64;
+                for(uint64_t u = first; u <= last; ++u)
+                    if(n_edges == 0 || staged[bounds_w + n_edges - 1] < u)
+                        staged[bounds_w + n_edges++] = u;
+            }
+            std::copy(bounds + g0, bounds + g0 + m + 1, staged.begin());
+            HIP(hipMemcpyAsync(d_bounds, staged.data(), (bounds_w + n_edges) * 8, hipMemcpyHostToDevice, stream));
             const uint64_t part_words = m * waves;      // u32 pairs: one word each
             hipLaunchKernelGGL(k_zero64, dim3(std::max<uint64_t>(1, std::min<uint64_t>((part_words + 255) / 256, 1024))), dim3(256), 0, stream,
                                reinterpret_cast<uint64_t*>(part), part_words);
             hipLaunchKernelGGL(k_zero64, dim3(1), dim3(256), 0, stream, edge, 2 * m);
             HIP(hipGetLastError());
-            const uint64_t tiles = (bounds[g0 + m] - (bounds[g0] & ~uint64_t(15)) + TILE - 1) / TILE;
+            const uint64_t tiles = (bounds[g0 + m] - origin + TILE - 1) / TILE;
             const uint32_t gr = static_cast<uint32_t>(std::min<uint64_t>(grid, std::max<uint64_t>(tiles, 1)));
             const uint64_t w = uint64_t(gr) * (TILE / 64);
             timed("query", [&]{
                 by_layout([&](auto L){
                     hipLaunchKernelGGL(k_count_batches<decltype(L)::value>, dim3(gr), dim3(TILE), 0, stream, view(), text,
-                                       static_cast<const uint64_t*>(d_bounds), static_cast<uint32_t>(m), part, edge);
+                                       static_cast<const uint64_t*>(d_bounds), static_cast<uint32_t>(m), part);
+                    if(n_edges)
+                        hipLaunchKernelGGL(k_count_edges<decltype(L)::value>, dim3(static_cast<uint32_t>(std::min<uint64_t>(n_edges, 2048))), dim3(64), 0,
+                                           stream, view(), text, static_cast<const uint64_t*>(d_bounds), static_cast<uint32_t>(m),
+                                           static_cast<const uint64_t*>(d_tiles), n_edges, edge);
                 });
             });
             hipLaunchKernelGGL(k_batch_sums, dim3(static_cast<uint32_t>(m)), dim3(256), 0, stream, static_cast<const uint32_t*>(part), w,
